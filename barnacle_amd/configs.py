@@ -1,0 +1,195 @@
+"""Seeded synthetic scenes in the reference's JSON schema.
+
+None of these scenes ships with the reference (its only scene is
+Asset/cbox.json); they are authored here from the workload descriptions:
+
+  cornell_box()        classic Cornell box: 5 axis-aligned wall quads and two
+                       rotated boxes (34 triangles) plus a 2-triangle area light
+                       of radiance (17,12,4); Lambertian red/green/white;
+                       256x256, fov 39.3, 16 spp, depth 8.
+  displaced_sphere()   geodesic icosphere of frequency 59 (20*59^2 = 69,620
+                       triangles; no icosphere has exactly 69,451 faces, which is
+                       the face count of the reference's bunny asset) displaced
+                       radially by 4-octave value noise (amplitude 0.08).
+  config1()            Cornell box + displaced sphere with one GGX (pbr)
+                       material drawn from seed 1; 1024x1024, 64 spp, depth 16.
+  caustic_scene()      Cornell box + dielectric sphere (IOR 1.5) and a GGX metal
+                       sphere (roughness 0.1) of 20,480 triangles each, PSSMLT.
+
+Geometry is generated in float64 and rounded once to float32.  The camera
+looks down -Z (Extensions/Camera/Pinhole.fs:21), so the box spans z in
+[-555, 0] and the camera sits at z = +800 with no rotation.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+WHITE = [0.73, 0.73, 0.73]
+RED = [0.63, 0.065, 0.05]
+GREEN = [0.14, 0.45, 0.09]
+
+
+def _quad(p0, p1, p2, p3):
+    v = np.array([p0, p1, p2, p3], np.float64).astype(np.float32)
+    return {"type": "mesh", "vertices": v.reshape(-1), "indices": np.array([0, 1, 2, 0, 2, 3], np.int32)}
+
+
+def icosphere(freq: int):
+    """Geodesic sphere: every icosahedron face split into freq^2 triangles,
+    vertices projected to the unit sphere, shared vertices merged, faces wound
+    so that cross(p1-p0, p2-p0) points outward."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    base = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                     [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    key_to_id, verts, tris = {}, [], []
+
+    def vid(p):
+        p = p / np.linalg.norm(p)
+        k = tuple(np.round(p * 1e9).astype(np.int64))
+        if k not in key_to_id:
+            key_to_id[k] = len(verts)
+            verts.append(p)
+        return key_to_id[k]
+
+    for a, b, c in faces:
+        A, B, Cc = base[a], base[b], base[c]
+        grid = {}
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                grid[i, j] = vid(A + (B - A) * (i / freq) + (Cc - A) * (j / freq))
+        for i in range(freq):
+            for j in range(freq - i):
+                tris.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                if i + j < freq - 1:
+                    tris.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+    V = np.array(verts)
+    T = np.array(tris, np.int64)
+    n = np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[T].mean(axis=1)) < 0
+    T[flip] = T[flip][:, [0, 2, 1]]
+    return V, T.astype(np.int32)
+
+
+def value_noise(p, seed: int, octaves: int = 4):
+    """Sum of `octaves` trilinear value-noise layers (frequency x2, amplitude
+    /2 per octave) over a 256-entry seeded lattice, normalised to [-1, 1]."""
+    rng = np.random.RandomState(seed)
+    perm = rng.permutation(256)
+    vals = rng.uniform(-1.0, 1.0, 256)
+
+    def lattice(ix, iy, iz):
+        return vals[perm[(perm[(perm[ix & 255] + iy) & 255] + iz) & 255]]
+
+    total, amp, freq, norm = np.zeros(len(p)), 1.0, 2.0, 0.0
+    for _ in range(octaves):
+        q = p * freq + 17.0
+        i = np.floor(q).astype(np.int64)
+        f = q - i
+        s = f * f * (3 - 2 * f)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    w = (s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) * \
+                        (s[:, 2] if dz else 1 - s[:, 2])
+                    acc += w * lattice(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz)
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+def displaced_sphere(freq: int = 59, amplitude: float = 0.08, seed: int = 1):
+    V, T = icosphere(freq)
+    V = V * (1.0 + amplitude * value_noise(V, seed))[:, None]
+    return {"type": "mesh", "vertices": V.astype(np.float32).reshape(-1), "indices": T.reshape(-1)}
+
+
+def plain_sphere(freq: int = 32):
+    V, T = icosphere(freq)
+    return {"type": "mesh", "vertices": V.astype(np.float32).reshape(-1), "indices": T.reshape(-1)}
+
+
+def cornell_box(width=256, height=256, spp=16, max_depth=8, boxes=True):
+    S = 555.0
+    prims = [
+        _quad((0, 0, 0), (S, 0, 0), (S, 0, -S), (0, 0, -S)),          # floor
+        _quad((0, S, 0), (0, S, -S), (S, S, -S), (S, S, 0)),          # ceiling
+        _quad((0, 0, -S), (S, 0, -S), (S, S, -S), (0, S, -S)),        # back
+        _quad((0, 0, 0), (0, 0, -S), (0, S, -S), (0, S, 0)),          # left (red)
+        _quad((S, 0, 0), (S, S, 0), (S, S, -S), (S, 0, -S)),          # right (green)
+        _quad((213, 554, -227), (343, 554, -227), (343, 554, -332), (213, 554, -332)),  # light
+    ]
+    mats = [{"type": "lambertian", "albedo": WHITE}, {"type": "lambertian", "albedo": RED},
+            {"type": "lambertian", "albedo": GREEN}]
+    insts = [{"primitive": 0, "material": 0}, {"primitive": 1, "material": 0},
+             {"primitive": 2, "material": 0}, {"primitive": 3, "material": 1},
+             {"primitive": 4, "material": 2}, {"primitive": 5, "light": 0}]
+    nodes = [{"instances": list(range(6)), "children": []}]
+    transforms = []
+    if boxes:
+        prims.append({"type": "cube"})
+        transforms += [
+            {"keyframes": [{"time": 0.0, "scale": [82.5, 82.5, 82.5], "rotation": [0.0, 0.3141593, 0.0],
+                            "translation": [185.5, 82.5, -168.5]}]},
+            {"keyframes": [{"time": 0.0, "scale": [82.5, 165.0, 82.5], "rotation": [0.0, -0.296706, 0.0],
+                            "translation": [368.5, 165.0, -351.5]}]}]
+        insts += [{"primitive": 6, "material": 0}, {"primitive": 6, "material": 0}]
+        nodes += [{"instances": [6], "transform": 0}, {"instances": [7], "transform": 1}]
+    transforms.append({"keyframes": [{"time": 0.0, "translation": [277.5, 277.5, 800.0]}]})
+    nodes.append({"transform": len(transforms) - 1, "has-camera": True})
+    nodes[0]["children"] = list(range(1, len(nodes)))
+    return {
+        "root": 0, "nodes": nodes, "instances": insts, "transforms": transforms, "primitives": prims,
+        "materials": mats, "lights": [{"type": "diffuse", "emission": [17.0, 12.0, 4.0]}],
+        "integrator": {"type": "path-tracing", "spp": spp, "max-depth": max_depth, "rr-depth": 5},
+        "camera": {"type": "pinhole", "fov": 39.3, "aspect-ratio": 1.0},
+        "film": {"width": width, "height": height, "tone-mapping": "aces"},
+    }
+
+
+def _add_mesh_instance(info, prim, material, scale, translation):
+    info["primitives"].append(prim)
+    info["materials"].append(material)
+    info["instances"].append({"primitive": len(info["primitives"]) - 1,
+                              "material": len(info["materials"]) - 1})
+    info["transforms"].append({"keyframes": [{"time": 0.0, "scale": [scale] * 3,
+                                              "translation": list(translation)}]})
+    info["nodes"].append({"instances": [len(info["instances"]) - 1],
+                          "transform": len(info["transforms"]) - 1})
+    info["nodes"][0]["children"].append(len(info["nodes"]) - 1)
+
+
+def config0():
+    return cornell_box()
+
+
+def config1(width=1024, height=1024, spp=64, max_depth=16, freq=59):
+    info = cornell_box(width, height, spp, max_depth)
+    rng = np.random.default_rng(1)
+    base = [float(np.float32(x)) for x in rng.uniform(0.2, 0.9, 3)]
+    metallic = float(rng.integers(0, 2))
+    rough = float(np.float32(rng.uniform(0.05, 0.8)))
+    _add_mesh_instance(info, displaced_sphere(freq, 0.08, 1),
+                       {"type": "pbr", "albedo": base, "metallic": metallic, "roughness": rough},
+                       90.0, (185.5, 300.0, -168.5))
+    return info
+
+
+def caustic_scene(width=1024, height=1024, spp=256, max_depth=16, freq=32,
+                  n_bootstrap=1 << 20, n_chains=1 << 20, sigma=1.0 / 64.0, large_step=0.3):
+    info = cornell_box(width, height, spp, max_depth, boxes=False)
+    _add_mesh_instance(info, plain_sphere(freq), {"type": "dielectric", "ior": 1.5},
+                       110.0, (380.0, 110.0, -200.0))
+    _add_mesh_instance(info, plain_sphere(freq), {"type": "pbr", "albedo": [0.9, 0.9, 0.9],
+                                                  "metallic": 1.0, "roughness": 0.1},
+                       110.0, (160.0, 110.0, -380.0))
+    info["integrator"] = {"type": "pssmlt", "spp": spp, "max-depth": max_depth, "rr-depth": 5,
+                          "n-bootstrap": n_bootstrap, "n-chains": n_chains,
+                          "mutation-strategy": "Gaussian", "mutation-sigma": sigma,
+                          "large-step-prob": large_step}
+    return info

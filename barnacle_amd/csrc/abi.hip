@@ -1,0 +1,688 @@
+// abi.hip -- the extern "C" entry points of include/barnacle_hip.h.
+//
+// Host side of the drop-in boundary: owns device memory and the stream,
+// uploads scenes in the SoA layout the kernels read, and drives the bounce
+// loop.  Errors become status codes with a thread-local message; no C++
+// exception crosses the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/barnacle_hip.h"
+#include "abi_internal.hpp"
+#include "host_scene.hpp"
+#include "kernels.hpp"
+
+using bnh::Error;
+
+namespace {
+thread_local std::string g_error;
+
+int guard(const std::function<void()>& f) {
+    try {
+        f();
+        return BN_OK;
+    } catch (const Error& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return BN_E_INVALID;
+    }
+}
+}  // namespace
+
+namespace bnx {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(BN_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::~DevBuf() { if (p) (void)hipFree(p); }
+void DevBuf::ensure(size_t n) {
+    if (n <= bytes) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+}
+void DevBuf::upload(const void* src, size_t n, hipStream_t s) {
+    ensure(std::max<size_t>(n, 16));
+    if (n) hip_check(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+hipEvent_t KernelTimer::next() {
+    if (used == pool.size()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+void KernelTimer::begin(const char* name, hipStream_t s) {
+    if (!on) return;
+    pending.push_back({name, used});
+    hip_check(hipEventRecord(next(), s), "hipEventRecord");
+}
+void KernelTimer::end(hipStream_t s) {
+    if (!on) return;
+    hip_check(hipEventRecord(next(), s), "hipEventRecord");
+}
+void KernelTimer::resolve() {
+    for (const auto& pr : pending) {
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, pool[pr.second], pool[pr.second + 1]), "hipEventElapsedTime");
+        bool found = false;
+        for (auto& t : totals)
+            if (t.first == pr.first) { t.second.first += 1; t.second.second += ms; found = true; break; }
+        if (!found) totals.push_back({pr.first, {1, (double)ms}});
+    }
+    pending.clear();
+    used = 0;
+}
+KernelTimer::~KernelTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+// Brackets one kernel launch with timing events when timing is on.
+struct Timed {
+    KernelTimer& t;
+    hipStream_t s;
+    Timed(KernelTimer& timer, const char* name, hipStream_t st) : t(timer), s(st) { t.begin(name, s); }
+    ~Timed() { t.end(s); }
+};
+
+static bnd::Xf to_xf(const float* m) {
+    bnd::Xf x;
+    x.c0 = make_float4(m[0], m[4], m[8], m[12]);
+    x.c1 = make_float4(m[1], m[5], m[9], m[13]);
+    x.c2 = make_float4(m[2], m[6], m[10], m[14]);
+    return x;
+}
+
+// A tree the kernels may traverse: interior children at i+1 and at a larger
+// index, leaves inside [0, n_items), depth <= 64 (BVHBuildConfig.MaxDepth),
+// which bounds the traversal stack (kStackMax) and rules out cycles.
+void validate_bvh(const std::vector<bn_bvh_node>& nodes, int n_items, const char* what) {
+    const int n = (int)nodes.size();
+    if (n == 0) throw Error(BN_E_INVALID, std::string(what) + ": empty BVH");
+    std::vector<std::pair<int, int>> work{{0, 0}};
+    int64_t seen = 0;
+    while (!work.empty()) {
+        auto [i, depth] = work.back();
+        work.pop_back();
+        if (i < 0 || i >= n || depth > 64 || ++seen > n)
+            throw Error(BN_E_INVALID, std::string(what) + ": malformed BVH");
+        const bn_bvh_node& b = nodes[i];
+        if (b.is_leaf) {
+            if (b.right_or_offset < 0 || b.count < 0 || (int64_t)b.right_or_offset + b.count > n_items)
+                throw Error(BN_E_INVALID, std::string(what) + ": BVH leaf outside the item range");
+        } else {
+            if (b.split_axis < 0 || b.split_axis > 2 || b.right_or_offset <= i + 1)
+                throw Error(BN_E_INVALID, std::string(what) + ": malformed BVH");
+            work.push_back({i + 1, depth + 1});
+            work.push_back({b.right_or_offset, depth + 1});
+        }
+    }
+}
+
+// Concatenate the referenced BLAS and upload the whole scene.
+void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
+                  const std::vector<bn_instance_upload>& insts, const std::vector<bn_material>& mats,
+                  const std::vector<bn_light>& lights, const bn_camera& cam, const float* c2w) {
+    for (const auto& in : insts) {
+        if (in.blas >= (int)c->blas.size() || (in.blas < 0 && !(in.sphere_radius == in.sphere_radius)))
+            throw Error(BN_E_INVALID, "instance references an unknown BLAS");
+        if (in.material >= (int)mats.size() || in.light >= (int)lights.size())
+            throw Error(BN_E_INVALID, "instance references an unknown material/light");
+    }
+    if (tlas.empty()) throw Error(BN_E_INVALID, "empty TLAS");
+    validate_bvh(tlas, (int)insts.size(), "TLAS");
+    // BLAS pool: nodes keep their BLAS-local child indices; each instance
+    // records where its BLAS starts in the node and triangle pools.
+    std::vector<int> node_base(c->blas.size(), -1), tri_base(c->blas.size(), -1);
+    std::vector<bn_bvh_node> nodes;
+    std::vector<float> tris, aprob, apdf;
+    std::vector<int> aidx;
+    for (const auto& in : insts) {
+        if (in.blas < 0 || node_base[in.blas] >= 0) continue;
+        const BlasRec& b = c->blas[in.blas];
+        node_base[in.blas] = (int)nodes.size();
+        tri_base[in.blas] = (int)(tris.size() / 9);
+        nodes.insert(nodes.end(), b.nodes.begin(), b.nodes.end());
+        tris.insert(tris.end(), b.tris.begin(), b.tris.end());
+        aprob.insert(aprob.end(), b.alias.prob.begin(), b.alias.prob.end());
+        apdf.insert(apdf.end(), b.alias.pdf.begin(), b.alias.pdf.end());
+        aidx.insert(aidx.end(), b.alias.alias.begin(), b.alias.alias.end());
+    }
+    std::vector<bnd::DevInst> di(insts.size());
+    std::vector<int4> lrec;
+    for (size_t i = 0; i < insts.size(); ++i) {
+        const bn_instance_upload& in = insts[i];
+        bnd::DevInst& d = di[i];
+        d.w2o = to_xf(in.w2o);
+        d.o2w = to_xf(in.o2w);
+        float pdf_const = 0.0f;
+        int ntri = 0;
+        if (in.blas >= 0) {
+            const BlasRec& b = c->blas[in.blas];
+            ntri = (int)(b.tris.size() / 9);
+            if (in.light >= 0) {
+                // MeshInstance.EvalPDF (Mesh.fs:300-304) with the tag reset to 0.
+                bnh::M4 m;
+                std::memcpy(m.m, in.o2w, 64);
+                const float* p = b.tris.data();
+                bnh::V3 p0 = bnh::transform_point({p[0], p[1], p[2]}, m);
+                bnh::V3 p1 = bnh::transform_point({p[3], p[4], p[5]}, m);
+                bnh::V3 p2 = bnh::transform_point({p[6], p[7], p[8]}, m);
+                pdf_const = b.alias.pdf[0] / bnh::tri_area(p0, p1, p2);
+            }
+        }
+        d.lo = make_float4(in.bounds[0], in.bounds[1], in.bounds[2], in.sphere_radius);
+        d.hi = make_float4(in.bounds[3], in.bounds[4], in.bounds[5], pdf_const);
+        d.info = make_int4(in.blas >= 0 ? node_base[in.blas] : -1, in.blas >= 0 ? tri_base[in.blas] : 0,
+                           in.material, in.light);
+        if (in.light >= 0) lrec.push_back(make_int4((int)i, d.info.y, ntri, in.light));
+    }
+    std::vector<float4> dmats(2 * std::max<size_t>(1, mats.size()));
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const bn_material& m = mats[i];
+        if (m.kind < 0 || m.kind > 3) throw Error(BN_E_INVALID, "unknown material kind");
+        int kind = m.kind;
+        float kf;
+        std::memcpy(&kf, &kind, 4);
+        // PBRMaterial: Alpha = Max(roughness^2, 1e-3), Metallic = Clamp(m, 0, 1) (PBR.fs:86-87)
+        float r2 = m.roughness * m.roughness;
+        float alpha = (r2 != r2) ? r2 : (r2 > 1e-3f ? r2 : 1e-3f);
+        float met = m.metallic < 0.0f ? 0.0f : (m.metallic > 1.0f ? 1.0f : m.metallic);
+        dmats[2 * i] = make_float4(kf, m.albedo[0], m.albedo[1], m.albedo[2]);
+        dmats[2 * i + 1] = make_float4(m.ior, alpha, met, 0.0f);
+    }
+    std::vector<float4> dl(std::max<size_t>(1, lights.size()));
+    for (size_t i = 0; i < lights.size(); ++i)
+        dl[i] = make_float4(lights[i].emission[0], lights[i].emission[1], lights[i].emission[2],
+                            lights[i].two_sided ? 1.0f : 0.0f);
+    hipStream_t s = c->stream;
+    sc.tlas.upload(tlas.data(), tlas.size() * sizeof(bn_bvh_node), s);
+    sc.blas.upload(nodes.data(), nodes.size() * sizeof(bn_bvh_node), s);
+    sc.tris.upload(tris.data(), tris.size() * sizeof(float), s);
+    sc.insts.upload(di.data(), di.size() * sizeof(bnd::DevInst), s);
+    sc.mats.upload(dmats.data(), dmats.size() * sizeof(float4), s);
+    sc.lightdefs.upload(dl.data(), dl.size() * sizeof(float4), s);
+    sc.lrec.upload(lrec.data(), lrec.size() * sizeof(int4), s);
+    sc.aprob.upload(aprob.data(), aprob.size() * sizeof(float), s);
+    sc.apdf.upload(apdf.data(), apdf.size() * sizeof(float), s);
+    sc.aidx.upload(aidx.data(), aidx.size() * sizeof(int), s);
+    hip_check(hipStreamSynchronize(s), "upload");
+    sc.n_lights = (int)lrec.size();
+    sc.n_insts = (int)insts.size();
+    sc.camera = cam;
+    std::memcpy(sc.c2w, c2w, 64);
+    sc.dev = bnd::DevScene{};
+    sc.dev.tlas = (const float4*)sc.tlas.p;
+    sc.dev.blas = (const float4*)sc.blas.p;
+    sc.dev.tris = (const float*)sc.tris.p;
+    sc.dev.insts = (const bnd::DevInst*)sc.insts.p;
+    sc.dev.mats = (const float4*)sc.mats.p;
+    sc.dev.lightdefs = (const float4*)sc.lightdefs.p;
+    sc.dev.light_insts = nullptr;
+    sc.dev.n_lights = sc.n_lights;
+    sc.dev.alias_prob = (const float*)sc.aprob.p;
+    sc.dev.alias_idx = (const int*)sc.aidx.p;
+    sc.dev.alias_pdf = (const float*)sc.apdf.p;
+    sc.dev.c2w = to_xf(c2w);
+    sc.dev.cam_kind = cam.kind;
+    sc.live = true;
+}
+
+// Viewport of PinholeCamera.GenerateRay (Pinhole.fs:12-21) for a film size.
+void camera_constants(SceneRec& sc, int W, int H) {
+    const bn_camera& c = sc.camera;
+    const float vh = 2.0f * (float)std::tan((double)(c.fov * bnh::kPi / 360.0f));
+    const float vw = vh * c.aspect;
+    const bnh::V3 U = vw * bnh::V3{1, 0, 0}, V = vh * bnh::V3{0, 1, 0};
+    const bnh::V3 du = U / (float)W, dv = V / (float)H;
+    const bnh::V3 ul = -bnh::V3{0, 0, 1} - 0.5f * (U + V);
+    sc.dev.ul = make_float4(ul.x, ul.y, ul.z, c.push_forward);
+    sc.dev.du = make_float4(du.x, du.y, du.z, c.aperture);
+    sc.dev.dv = make_float4(dv.x, dv.y, dv.z, c.focus_distance);
+}
+
+SceneRec& get_scene(Ctx* c, int id) {
+    if (!c) throw Error(BN_E_INVALID, "null context");
+    if (id < 0 || id >= (int)c->scenes.size() || !c->scenes[id]->live) throw Error(BN_E_INVALID, "unknown scene id");
+    return *c->scenes[id];
+}
+
+int grid_trace(Ctx* c) { return c->n_cu * c->trace_blocks_per_cu; }
+
+void ensure_paths(Ctx* c, size_t cap) {
+    if (cap <= c->path_cap) return;
+    auto f4 = [&](DevBuf& b) { b.ensure(cap * sizeof(float4)); };
+    f4(c->p_o); f4(c->p_d); f4(c->p_beta); f4(c->p_L); f4(c->p_sh_o); f4(c->p_sh_d); f4(c->p_nee_a); f4(c->p_nee_b);
+    c->p_rng.ensure(cap * 4);
+    c->p_hit.ensure(cap * sizeof(int4));
+    c->queue[0].ensure(cap * 4);
+    c->queue[1].ensure(cap * 4);
+    c->shq.ensure(cap * 4);
+    c->bins.ensure(cap * 4 * bnd::kBins);
+    c->path_cap = cap;
+}
+
+bnd::PathBuf path_buf(Ctx* c) {
+    bnd::PathBuf P;
+    P.o = (float4*)c->p_o.p; P.d = (float4*)c->p_d.p; P.beta = (float4*)c->p_beta.p; P.L = (float4*)c->p_L.p;
+    P.rng = (uint32_t*)c->p_rng.p; P.hit = (int4*)c->p_hit.p;
+    P.sh_o = (float4*)c->p_sh_o.p; P.sh_d = (float4*)c->p_sh_d.p;
+    P.nee_a = (float4*)c->p_nee_a.p; P.nee_b = (float4*)c->p_nee_b.p;
+    return P;
+}
+
+void ensure_spill(Ctx* c) {
+    const size_t lanes = (size_t)grid_trace(c) * bnd::kBlock;
+    c->spill.ensure(lanes * (bnd::kStackMax - bnd::kStackK) * 4);
+    c->spill_stride = lanes;
+}
+
+// The bounce loop of one frame of PathTracingIntegrator (or the Normal view).
+void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, float* film_host, bn_trace_stats* stats) {
+    if (p.width <= 0 || p.height <= 0) throw Error(BN_E_INVALID, "film size must be positive");
+    if (p.spp < 0 || p.max_depth < 0) throw Error(BN_E_INVALID, "spp and max-depth must be >= 0");
+    if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
+    hipStream_t s = c->stream;
+    camera_constants(sc, p.width, p.height);
+    bnd::ChunkInfo ci{};
+    ci.width = p.width; ci.height = p.height;
+    ci.tiles_x = (p.width + 7) / 8; ci.tiles_y = (p.height + 7) / 8;
+    ci.p_pad = ci.tiles_x * ci.tiles_y * 64;
+    ci.spp = p.spp; ci.frame_id = p.frame_id;
+    const size_t want = std::max<size_t>((size_t)ci.p_pad, c->chunk_slots);
+    const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / ci.p_pad));
+    ensure_paths(c, (size_t)ns * ci.p_pad);
+    ensure_spill(c);
+    const size_t film_n = (size_t)p.width * p.height * 3;
+    c->film.ensure(film_n * 4);
+    c->ctl.ensure(sizeof(bnd::Ctl));
+    c->stats.ensure(8 * sizeof(unsigned long long));
+    hip_check(hipMemsetAsync(c->film.p, 0, film_n * 4, s), "memset film");
+    hip_check(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), s), "memset stats");
+    const bool count = stats != nullptr;
+    bn_trace_stats acc{};
+    bnd::PathBuf P = path_buf(c);
+    bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
+    const uint32_t bin_cap = (uint32_t)(ns * ci.p_pad);
+    const float inv_spp = 1.0f / (float)p.spp;
+    const int gt = grid_trace(c);
+    const int gs = c->n_cu * 8;
+    for (int s0 = 0; s0 < p.spp; s0 += ns) {
+        ci.s0 = s0;
+        ci.ns = std::min(ns, p.spp - s0);
+        hip_check(hipMemsetAsync(ctl, 0, sizeof(bnd::Ctl), s), "memset ctl");
+        const int nslots = ci.ns * ci.p_pad;
+        KernelTimer& T = c->timer;
+        { Timed tm(T, "raygen", s);
+          bnd::launch_raygen(sc.dev, P, ctl, (uint32_t*)c->queue[0].p, ci, std::min(gs, (nslots + 255) / 256), s); }
+        int cur = 0;
+        const int depth_n = normal ? std::min(1, p.max_depth) : p.max_depth;
+        for (int depth = 0; depth < depth_n; ++depth) {
+            const int nxt = 1 - cur;
+            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
+            { Timed tm(T, "trace_closest", s);
+              bnd::launch_trace_closest(count, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
+                                        (uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->spill.p, c->spill_stride,
+                                        (unsigned long long*)c->stats.p, gt, s); }
+            if (normal) {
+                Timed tm(T, "shade_normal", s);
+                bnd::launch_shade_normal(sc.dev, P, ctl, (const uint32_t*)c->bins.p, bin_cap, gs, s);
+                break;
+            }
+            { Timed tm(T, "shade", s);
+              bnd::launch_shade(sc.dev, P, ctl, (const uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->queue[nxt].p,
+                                nxt, (uint32_t*)c->shq.p, (const int4*)sc.lrec.p, depth, p.max_depth, p.rr_depth,
+                                gs, s); }
+            { Timed tm(T, "trace_shadow", s);
+              bnd::launch_trace_shadow(count, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
+                                       c->spill_stride, (unsigned long long*)c->stats.p, gt, s); }
+            if (count) {
+                bnd::Ctl h;
+                hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "ctl readback");
+                hip_check(hipStreamSynchronize(s), "sync");
+                acc.rays_closest += h.q_count[cur];
+                acc.rays_shadow += h.sh_count;
+                for (int b = 0; b < bnd::kBins; ++b) acc.vertices += h.bin_count[b];
+            }
+            cur = nxt;
+        }
+        { Timed tm(T, "film", s); bnd::launch_film(P, (float*)c->film.p, ci, inv_spp, s); }
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+    if (film_host) hip_check(hipMemcpyAsync(film_host, c->film.p, film_n * 4, hipMemcpyDeviceToHost, s), "film D2H");
+    hip_check(hipStreamSynchronize(s), "render");
+    c->timer.resolve();
+    if (count) {
+        unsigned long long h[8];
+        hip_check(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost), "stats D2H");
+        acc.nodes_closest = (int64_t)h[0]; acc.tris_closest = (int64_t)h[1]; acc.insts_closest = (int64_t)h[2];
+        acc.nodes_shadow = (int64_t)h[3]; acc.tris_shadow = (int64_t)h[4]; acc.insts_shadow = (int64_t)h[5];
+        acc.paths = (int64_t)p.width * p.height * p.spp;
+        *stats = acc;
+    }
+}
+
+}  // namespace bnx
+
+using namespace bnx;
+
+extern "C" {
+
+const char* bn_last_error(void) { return g_error.c_str(); }
+int bn_abi_version(void) { return 1; }
+
+int bn_struct_sizes(int32_t* out, int32_t n) {
+    const int32_t v[] = {(int32_t)sizeof(bn_primitive), (int32_t)sizeof(bn_material), (int32_t)sizeof(bn_light),
+                         (int32_t)sizeof(bn_instance), (int32_t)sizeof(bn_transform), (int32_t)sizeof(bn_node),
+                         (int32_t)sizeof(bn_camera), (int32_t)sizeof(bn_scene_desc), (int32_t)sizeof(bn_pt_params),
+                         (int32_t)sizeof(bn_mlt_params), (int32_t)sizeof(bn_mlt_stats),
+                         (int32_t)sizeof(bn_trace_stats), (int32_t)sizeof(bn_bvh_node),
+                         (int32_t)sizeof(bn_instance_upload)};
+    const int m = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
+    return m;
+}
+
+int bn_create(int device, bn_ctx** out) {
+    return guard([&] {
+        if (!out) throw Error(BN_E_INVALID, "null output pointer");
+        int n = 0;
+        hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device < 0 || device >= n) throw Error(BN_E_DEVICE, "no such HIP device");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        auto c = std::make_unique<Ctx>();
+        c->device = device;
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        c->n_cu = prop.multiProcessorCount;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        *out = reinterpret_cast<bn_ctx*>(c.release());
+    });
+}
+
+void bn_destroy(bn_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+int bn_bvh_build(const float* b, int32_t n, bn_bvh_node* out_nodes, int32_t* out_n_nodes, int32_t* out_perm) {
+    return guard([&] {
+        if (n <= 0 || !b || !out_nodes || !out_n_nodes || !out_perm) throw Error(BN_E_INVALID, "bad arguments");
+        std::vector<bnh::Box> items(n);
+        for (int i = 0; i < n; ++i)
+            items[i] = {{b[6 * i], b[6 * i + 1], b[6 * i + 2]}, {b[6 * i + 3], b[6 * i + 4], b[6 * i + 5]}};
+        std::vector<bn_bvh_node> nodes;
+        std::vector<int32_t> perm;
+        bnh::build_bvh(items, nodes, perm);
+        if ((int)nodes.size() > *out_n_nodes) throw Error(BN_E_CAPACITY, "node buffer too small");
+        std::memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(bn_bvh_node));
+        *out_n_nodes = (int)nodes.size();
+        std::memcpy(out_perm, perm.data(), perm.size() * 4);
+    });
+}
+
+int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes, const float* verts, int32_t n_verts,
+                   const int32_t* tri_idx, int32_t n_tris, const float* alias_pdf, int32_t* out_blas) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || !nodes || n_nodes <= 0 || !verts || !tri_idx || n_tris <= 0 || !out_blas)
+            throw Error(BN_E_INVALID, "bad arguments");
+        BlasRec b;
+        b.nodes.assign(nodes, nodes + n_nodes);
+        b.tris.resize(9 * (size_t)n_tris);
+        for (int i = 0; i < n_tris; ++i)
+            for (int j = 0; j < 3; ++j) {
+                int v = tri_idx[3 * i + j];
+                if (v < 0 || v >= n_verts) throw Error(BN_E_INVALID, "triangle index out of range");
+                for (int k = 0; k < 3; ++k) b.tris[9 * (size_t)i + 3 * j + k] = verts[3 * (size_t)v + k];
+            }
+        std::vector<float> w(n_tris);
+        if (alias_pdf) {
+            // The F# host passes the alias table pdfs; the table itself is the
+            // degenerate construction over them (AliasTable.fs:14-51).
+            b.alias = bnh::build_alias(alias_pdf, n_tris);
+        } else {
+            for (int i = 0; i < n_tris; ++i) {
+                const float* t = &b.tris[9 * (size_t)i];
+                w[i] = bnh::tri_area({t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]});
+            }
+            b.alias = bnh::build_alias(w.data(), n_tris);
+        }
+        validate_bvh(b.nodes, n_tris, "BLAS");
+        c->blas.push_back(std::move(b));
+        *out_blas = (int)c->blas.size() - 1;
+    });
+}
+
+int bn_upload_scene(bn_ctx* ctx, const bn_bvh_node* tlas, int32_t n_tlas, const bn_instance_upload* instances,
+                    int32_t n_instances, const bn_material* materials, int32_t n_materials, const bn_light* lights,
+                    int32_t n_lights, const bn_camera* camera, const float* c2w, int32_t* out_scene) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || !tlas || n_tlas <= 0 || !instances || n_instances <= 0 || !camera || !c2w || !out_scene)
+            throw Error(BN_E_INVALID, "bad arguments");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        auto sc = std::make_unique<SceneRec>();
+        upload_scene(c, *sc, std::vector<bn_bvh_node>(tlas, tlas + n_tlas),
+                     std::vector<bn_instance_upload>(instances, instances + n_instances),
+                     std::vector<bn_material>(materials, materials + n_materials),
+                     std::vector<bn_light>(lights, lights + n_lights), *camera, c2w);
+        c->scenes.push_back(std::move(sc));
+        *out_scene = (int)c->scenes.size() - 1;
+    });
+}
+
+int bn_scene_build(bn_ctx* ctx, const bn_scene_desc* desc, int32_t* out_scene) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || !desc || !out_scene) throw Error(BN_E_INVALID, "bad arguments");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        auto hs = std::make_shared<bnh::HostScene>();
+        bnh::build_scene(*desc, *hs);
+        std::vector<int> blas_of(hs->meshes.size(), -1);
+        for (size_t i = 0; i < hs->meshes.size(); ++i) {
+            const bnh::HostMesh& m = hs->meshes[i];
+            if (m.sphere) continue;
+            BlasRec b;
+            b.nodes = m.nodes;
+            const int nt = (int)(m.tri.size() / 3);
+            b.tris.resize(9 * (size_t)nt);
+            for (int k = 0; k < nt; ++k)
+                for (int j = 0; j < 3; ++j) {
+                    const bnh::V3 v = m.verts[m.tri[3 * k + j]];
+                    b.tris[9 * (size_t)k + 3 * j] = v.x;
+                    b.tris[9 * (size_t)k + 3 * j + 1] = v.y;
+                    b.tris[9 * (size_t)k + 3 * j + 2] = v.z;
+                }
+            b.alias = m.alias;
+            validate_bvh(b.nodes, nt, "BLAS");
+            c->blas.push_back(std::move(b));
+            blas_of[i] = (int)c->blas.size() - 1;
+        }
+        std::vector<bn_instance_upload> ups(hs->insts.size());
+        for (size_t i = 0; i < hs->insts.size(); ++i) {
+            const bnh::HostInstance& h = hs->insts[i];
+            bn_instance_upload& u = ups[i];
+            std::memcpy(u.o2w, h.o2w.m, 64);
+            std::memcpy(u.w2o, h.w2o.m, 64);
+            u.bounds[0] = h.bounds.lo.x; u.bounds[1] = h.bounds.lo.y; u.bounds[2] = h.bounds.lo.z;
+            u.bounds[3] = h.bounds.hi.x; u.bounds[4] = h.bounds.hi.y; u.bounds[5] = h.bounds.hi.z;
+            u.blas = blas_of[h.prim];
+            u.sphere_radius = hs->meshes[h.prim].sphere ? hs->meshes[h.prim].radius : 0.0f;
+            u.material = h.material;
+            u.light = h.light;
+        }
+        auto sc = std::make_unique<SceneRec>();
+        upload_scene(c, *sc, hs->tlas, ups, hs->materials, hs->light_defs, hs->camera, hs->c2w.m);
+        sc->host = hs;
+        c->scenes.push_back(std::move(sc));
+        *out_scene = (int)c->scenes.size() - 1;
+    });
+}
+
+int bn_scene_release(bn_ctx* ctx, int32_t scene) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->scenes[scene] = std::make_unique<SceneRec>();
+        (void)s;
+    });
+}
+
+int bn_scene_export(bn_ctx* ctx, int32_t scene, bn_bvh_node* tlas, int32_t* n_tlas, int32_t* inst_order,
+                    int32_t* n_inst, float* o2w, float* w2o, int32_t* light_insts, int32_t* n_lights,
+                    float* c2w) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        SceneRec& s = get_scene(c, scene);
+        if (!s.host) throw Error(BN_E_INVALID, "scene was not built by bn_scene_build");
+        const bnh::HostScene& h = *s.host;
+        if ((int)h.tlas.size() > *n_tlas || (int)h.insts.size() > *n_inst || (int)h.lights.size() > *n_lights)
+            throw Error(BN_E_CAPACITY, "export buffer too small");
+        std::memcpy(tlas, h.tlas.data(), h.tlas.size() * sizeof(bn_bvh_node));
+        *n_tlas = (int)h.tlas.size();
+        for (size_t i = 0; i < h.insts.size(); ++i) {
+            inst_order[i] = h.inst_desc[i];
+            std::memcpy(o2w + 16 * i, h.insts[i].o2w.m, 64);
+            std::memcpy(w2o + 16 * i, h.insts[i].w2o.m, 64);
+        }
+        *n_inst = (int)h.insts.size();
+        std::memcpy(light_insts, h.lights.data(), h.lights.size() * 4);
+        *n_lights = (int)h.lights.size();
+        std::memcpy(c2w, h.c2w.m, 64);
+    });
+}
+
+int bn_scene_export_blas(bn_ctx* ctx, int32_t scene, int32_t prim, bn_bvh_node* nodes, int32_t* n_nodes,
+                         int32_t* tri, int32_t* n_tris, float* pdf) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        SceneRec& s = get_scene(c, scene);
+        if (!s.host) throw Error(BN_E_INVALID, "scene was not built by bn_scene_build");
+        const bnh::HostScene& h = *s.host;
+        if (prim < 0 || prim >= (int)h.meshes.size() || h.meshes[prim].sphere)
+            throw Error(BN_E_INVALID, "not a mesh primitive");
+        const bnh::HostMesh& m = h.meshes[prim];
+        const int nt = (int)(m.tri.size() / 3);
+        if ((int)m.nodes.size() > *n_nodes || nt > *n_tris) throw Error(BN_E_CAPACITY, "export buffer too small");
+        std::memcpy(nodes, m.nodes.data(), m.nodes.size() * sizeof(bn_bvh_node));
+        *n_nodes = (int)m.nodes.size();
+        std::memcpy(tri, m.tri.data(), m.tri.size() * 4);
+        std::memcpy(pdf, m.alias.pdf.data(), nt * 4);
+        *n_tris = nt;
+    });
+}
+
+int bn_render_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, float* film, bn_trace_stats* stats) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!params) throw Error(BN_E_INVALID, "null params");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        render_frames(c, s, *params, false, film, stats);
+    });
+}
+
+int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, float* film) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!params) throw Error(BN_E_INVALID, "null params");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        render_frames(c, s, *params, true, film, nullptr);
+    });
+}
+
+int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* p, int32_t sample_id, int32_t* inst,
+                 int32_t* prim, float* t) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!p || !inst || !prim || !t) throw Error(BN_E_INVALID, "bad arguments");
+        if (p->width <= 0 || p->height <= 0) throw Error(BN_E_INVALID, "film size must be positive");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        camera_constants(s, p->width, p->height);
+        ensure_spill(c);
+        const size_t n = (size_t)p->width * p->height;
+        c->out_a.ensure(n * 4);
+        c->out_b.ensure(n * 4);
+        c->out_c.ensure(n * 4);
+        bnd::launch_first_hit(s.dev, p->width, p->height, sample_id, p->spp, p->frame_id, (int*)c->out_a.p,
+                              (int*)c->out_b.p, (float*)c->out_c.p, (uint32_t*)c->spill.p, c->spill_stride,
+                              grid_trace(c), c->stream);
+        hip_check(hipGetLastError(), "kernel launch");
+        hip_check(hipMemcpyAsync(inst, c->out_a.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(prim, c->out_b.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(t, c->out_c.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "first hit");
+    });
+}
+
+}  // extern "C"
+
+extern "C" int bn_render_pssmlt(bn_ctx* ctx, int32_t scene, const bn_mlt_params* params, float* film,
+                                bn_mlt_stats* stats) {
+    return guard([&] {
+        (void)ctx; (void)scene; (void)params; (void)film; (void)stats;
+        throw Error(BN_E_UNSUPPORTED, "bn_render_pssmlt: not built yet");
+    });
+}
+
+extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || !key) throw Error(BN_E_INVALID, "bad arguments");
+        const std::string k(key);
+        if (k == "timing") c->timer.on = value != 0;
+        else if (k == "chunk_slots") {
+            if (value < 64 || value > ((int64_t)1 << 30)) throw Error(BN_E_INVALID, "chunk_slots out of range");
+            c->chunk_slots = (size_t)value;
+        } else if (k == "trace_blocks_per_cu") {
+            if (value < 1 || value > 5) throw Error(BN_E_INVALID, "trace_blocks_per_cu must be in [1, 5]");
+            c->trace_blocks_per_cu = (int)value;
+        } else throw Error(BN_E_INVALID, "unknown option " + k);
+    });
+}
+
+extern "C" int bn_kernel_times(bn_ctx* ctx, bn_kernel_time* out, int32_t* n) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || !out || !n) throw Error(BN_E_INVALID, "bad arguments");
+        const auto& t = c->timer.totals;
+        if ((int)t.size() > *n) throw Error(BN_E_CAPACITY, "kernel time buffer too small");
+        for (size_t i = 0; i < t.size(); ++i) {
+            std::memset(out[i].name, 0, sizeof(out[i].name));
+            std::strncpy(out[i].name, t[i].first.c_str(), sizeof(out[i].name) - 1);
+            out[i].launches = t[i].second.first;
+            out[i].total_ms = t[i].second.second;
+        }
+        *n = (int)t.size();
+    });
+}
+
+extern "C" int bn_reset_kernel_times(bn_ctx* ctx) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c) throw Error(BN_E_INVALID, "null context");
+        c->timer.totals.clear();
+    });
+}

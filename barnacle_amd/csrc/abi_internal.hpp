@@ -1,0 +1,82 @@
+// abi_internal.hpp -- context and scene records behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/barnacle_hip.h"
+#include "host_scene.hpp"
+#include "kernels.hpp"
+
+namespace bnx {
+
+void hip_check(hipError_t e, const char* what);
+
+// Device allocation that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void ensure(size_t n);
+    void upload(const void* src, size_t n, hipStream_t s);
+};
+
+// A bottom-level acceleration structure as MeshPrimitive holds it
+// (Mesh.fs:157-186): nodes, triangles gathered in BVH order, area alias table.
+struct BlasRec {
+    std::vector<bn_bvh_node> nodes;
+    std::vector<float> tris;     // 9 floats per triangle
+    bnh::AliasTable alias;
+};
+
+struct SceneRec {
+    bool live = false;
+    DevBuf tlas, blas, tris, insts, mats, lightdefs, lrec, aprob, apdf, aidx;
+    int n_lights = 0, n_insts = 0;
+    bn_camera camera{};
+    float c2w[16] = {};
+    bnd::DevScene dev{};
+    std::shared_ptr<bnh::HostScene> host;   // set by bn_scene_build
+};
+
+// Event pairs around kernel launches, resolved after the stream syncs.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<std::string, size_t>> pending;   // name, first event index
+    std::vector<std::pair<std::string, std::pair<int64_t, double>>> totals;
+    hipEvent_t next();
+    void begin(const char* name, hipStream_t s);
+    void end(hipStream_t s);
+    void resolve();                                        // after a stream sync
+    ~KernelTimer();
+};
+
+struct Ctx {
+    int device = 0;
+    int n_cu = 1;
+    hipStream_t stream = nullptr;
+    int trace_blocks_per_cu = 4;
+    size_t chunk_slots = (size_t)8 << 20;
+    std::vector<BlasRec> blas;
+    std::vector<std::unique_ptr<SceneRec>> scenes;
+    // path state
+    size_t path_cap = 0;
+    DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b;
+    DevBuf queue[2], shq, bins, spill, film, ctl, stats;
+    size_t spill_stride = 0;
+    DevBuf out_a, out_b, out_c;
+    KernelTimer timer;
+    ~Ctx() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace bnx

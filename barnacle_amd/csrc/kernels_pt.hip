@@ -1,0 +1,357 @@
+// kernels_pt.hip -- wavefront path tracing on gfx950.
+//
+// One pass of PathTracingIntegrator.Li (Extensions/Integrator/PathTracing.fs:14-81)
+// is split into per-bounce kernels over compacted queues of path slots:
+//
+//   raygen -> [ reset -> trace_closest (bins by material) -> shade
+//               -> trace_shadow ] x max_depth -> film
+//
+// A path's state lives in SoA arrays indexed by its slot.  All paths that are
+// alive at bounce k have depth k, so depth is the loop index.  The L update
+// order of the reference (emission MIS, then NEE, per bounce) is kept because
+// trace_shadow of bounce k runs before shade of bounce k+1.  Radiance of each
+// (pixel, sample) is kept per slot and folded into the film in sample order by
+// k_film, which makes the film bit-identical to ProgressiveIntegrator.RenderTile
+// (Base/Integrator.fs:36-44).
+#include "kernels.hpp"
+#include "shading.hpp"
+
+namespace bnd {
+
+// ----------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Wave-uniform fetch of `n` work items: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* ctr, uint32_t n) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, n);
+    return __shfl(base, 0);
+}
+// Ballot-compacted append: one atomic per wave, lanes with pred write val.
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_push(bool pred, uint32_t* ctr, uint32_t* arr, uint32_t val) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) arr[base + mbcnt(m)] = val;
+}
+__device__ __forceinline__ void wave_add_stats(unsigned long long* dst, uint32_t v) {
+    // sum over the wave, one 64-bit atomic
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+// slot -> (pixel, sample) for a chunk: 8x8 pixel tiles, sample-major.
+__device__ __forceinline__ bool slot_pixel(const ChunkInfo& c, uint32_t slot, int& x, int& y, int& s) {
+    const uint32_t q = slot % (uint32_t)c.p_pad;
+    s = c.s0 + (int)(slot / (uint32_t)c.p_pad);
+    const uint32_t tile = q >> 6;
+    x = (int)((tile % (uint32_t)c.tiles_x) * 8 + (q & 7));
+    y = (int)((tile / (uint32_t)c.tiles_x) * 8 + ((q >> 3) & 7));
+    return x < c.width && y < c.height;
+}
+
+// ------------------------------------------------------------------ raygen
+// RenderTile seeding and primary ray (Base/Integrator.fs:34-39).
+__global__ __launch_bounds__(256) void k_raygen(DevScene S, PathBuf P, Ctl* ctl, uint32_t* queue, ChunkInfo c) {
+    const uint32_t n = (uint32_t)c.ns * (uint32_t)c.p_pad;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t slot = base + threadIdx.x;
+        int x = 0, y = 0, s = 0;
+        bool ok = slot < n && slot_pixel(c, slot, x, y, s);
+        if (ok) {
+            uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(c.frame_id * c.spp + s));
+            V2 uPix, uLens;
+            uPix.x = lcg(rng); uPix.y = lcg(rng);
+            uLens.x = lcg(rng); uLens.y = lcg(rng);
+            Ray r = camera_ray(S, x, y, uPix, uLens);
+            P.o[slot] = f4(r.o, 0.0f);
+            P.d[slot] = f4(r.d, 0.0f);
+            P.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            P.rng[slot] = rng;
+        }
+        wave_push(ok, &ctl->q_count[0], queue, slot);
+    }
+}
+
+// Per-bounce counter reset (kept as a kernel so the bounce loop can be
+// captured in a graph without memset nodes).
+__global__ void k_reset(Ctl* ctl, int nxt) {
+    const int i = threadIdx.x;
+    if (i < kBins) ctl->bin_count[i] = 0;
+    if (i == 0) { ctl->sh_count = 0; ctl->q_count[nxt] = 0; }
+    if (i < 4) ctl->fetch[i] = 0;
+}
+
+// ------------------------------------------------------------ trace closest
+// BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  Hits
+// are appended to the bin of their instance's material kind (light-only
+// instances to the last bin) so that shade runs over coherent waves.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_closest(DevScene S, PathBuf P, Ctl* ctl, int cur,
+                                                       const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
+                                                       uint32_t* spill, size_t spill_stride,
+                                                       unsigned long long* stats) {
+    __shared__ uint32_t stk[kStackK * 256];
+    const uint32_t count = ctl->q_count[cur];
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    TraceCount tc{0, 0, 0};
+    for (;;) {
+        const uint32_t base = wave_fetch(&ctl->fetch[0], 64);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        int bin = -1;
+        uint32_t slot = 0;
+        if (i < count) {
+            slot = queue[i];
+            Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
+            float t = kInf;
+            int hi = -1, hp = -1;
+            if (trace<false, COUNT>(S, r, t, hi, hp, st, tc)) {
+                P.hit[slot] = make_int4(hi, hp, __float_as_int(t), 0);
+                const int4 info = S.insts[hi].info;
+                bin = info.z >= 0 ? __float_as_int(S.mats[2 * info.z].x) : kBins - 1;
+            }
+        }
+        for (int b = 0; b < kBins; ++b) wave_push(bin == b, &ctl->bin_count[b], bins + (size_t)b * bin_cap, slot);
+    }
+    if (COUNT) {
+        wave_add_stats(&stats[0], tc.nodes);
+        wave_add_stats(&stats[1], tc.tris);
+        wave_add_stats(&stats[2], tc.insts);
+    }
+}
+
+// ------------------------------------------------------------------- shade
+// The body of one loop iteration of PathTracing.fs:24-79 after the closest
+// hit: emission with MIS (:30-40), next-event estimation setup (:42-50), BSDF
+// sampling (:61-67) and Russian roulette (:69-75).  The NEE contribution is
+// applied by k_trace_shadow when the shadow ray is unoccluded (:47-59).
+__global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
+                                               uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec,
+                                               int depth, int max_depth, int rr_depth) {
+    uint32_t start[kBins + 1];
+    start[0] = 0;
+    for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
+    const uint32_t total = start[kBins];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        uint32_t slot = 0;
+        if (i < total) {
+            int b = 0;
+            while (i >= start[b + 1]) ++b;
+            slot = bins[(size_t)b * bin_cap + (i - start[b])];
+            const float4 o4 = P.o[slot], d4 = P.d[slot];
+            const Ray r = {xyz(o4), xyz(d4)};
+            const float prev_pdf = d4.w;
+            const int4 h = P.hit[slot];
+            const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+            const int4 info = S.insts[h.x].info;
+            V3 L = xyz(P.L[slot]);
+            V3 beta = xyz(P.beta[slot]);
+            if (info.w >= 0) {
+                V3 Le;
+                float lpdf;
+                light_eval(S, h.x, r.o, g, Le, lpdf);
+                const float w = depth == 0 ? 1.0f : prev_pdf * rcp_e(lpdf + prev_pdf);
+                L = fma3(beta, Le * w, L);
+            }
+            if (info.z >= 0) {
+                uint32_t rng = P.rng[slot];
+                const Mat m = load_mat(S, info.z);
+                const V3 wo = -r.d;
+                const V3 wo_l = to_local(g, wo);
+                const float uSel = lcg(rng);
+                V2 uL;
+                uL.x = lcg(rng); uL.y = lcg(rng);
+                const LightSample ls = light_sample(S, lrec, g.p, uSel, uL);
+                if (ls.pdf != 0.0f) {
+                    const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
+                    const float dist = length(ls.p - g.p);
+                    P.sh_o[slot] = f4(g.p, dist - 1e-3f);
+                    P.sh_d[slot] = f4(ls.wi, 0.0f);
+                    P.nee_a[slot] = f4(beta * e.f, 0.0f);
+                    P.nee_b[slot] = f4(ls.L * rcp_e(e.pdf + ls.pdf), 0.0f);
+                    shadow = true;
+                }
+                const float ul = lcg(rng);
+                V2 ub;
+                ub.x = lcg(rng); ub.y = lcg(rng);
+                V3 wi_l;
+                const BSDF bs = mat_sample(m, wo_l, ul, ub, wi_l);
+                if (bs.pdf != 0.0f) {
+                    const V3 wi = to_world(g, wi_l);
+                    beta = beta * bs.f * rcp_e(bs.pdf);
+                    cont = true;
+                    if (depth >= rr_depth) {
+                        const float q = min_n(1.0f, max_n(beta.x, max_n(beta.y, beta.z)));
+                        if (lcg(rng) < q) beta = beta * rcp_e(q);
+                        else cont = false;
+                    }
+                    P.o[slot] = f4(g.p, 0.0f);
+                    P.d[slot] = f4(wi, bs.pdf);
+                    P.beta[slot] = f4(beta, 0.0f);
+                }
+                P.rng[slot] = rng;
+                cont = cont && depth + 1 < max_depth;
+            }
+            P.L[slot] = f4(L, 0.0f);
+        }
+        wave_push(cont, &ctl->q_count[nxt], queue_next, slot);
+        wave_push(shadow, &ctl->sh_count, shq, slot);
+    }
+}
+
+// ----------------------------------------------------------- trace shadow
+// aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
+// an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_shadow(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* shq,
+                                                      uint32_t* spill, size_t spill_stride,
+                                                      unsigned long long* stats) {
+    __shared__ uint32_t stk[kStackK * 256];
+    const uint32_t count = ctl->sh_count;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    TraceCount tc{0, 0, 0};
+    for (;;) {
+        const uint32_t base = wave_fetch(&ctl->fetch[1], 64);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i < count) {
+            const uint32_t slot = shq[i];
+            const float4 o4 = P.sh_o[slot];
+            const Ray r = {xyz(o4), xyz(P.sh_d[slot])};
+            float t = o4.w;
+            int hi, hp;
+            if (!trace<true, COUNT>(S, r, t, hi, hp, st, tc)) {
+                const V3 a = xyz(P.nee_a[slot]), b = xyz(P.nee_b[slot]);
+                P.L[slot] = f4(fma3(a, b, xyz(P.L[slot])), 0.0f);
+            }
+        }
+    }
+    if (COUNT) {
+        wave_add_stats(&stats[3], tc.nodes);
+        wave_add_stats(&stats[4], tc.tris);
+        wave_add_stats(&stats[5], tc.insts);
+    }
+}
+
+// -------------------------------------------------------------------- film
+// accum = FMA(1/spp, Li / pdf, accum) in sample order (Integrator.fs:41-42);
+// the camera pdf is 1.  The film keeps Film.Pixels order (Film.fs:43).
+__global__ void k_film(PathBuf P, float* film, ChunkInfo c, float inv_spp) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= c.width || y >= c.height) return;
+    const uint32_t q = ((uint32_t)((y >> 3) * c.tiles_x + (x >> 3)) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
+    const size_t fi = 3 * ((size_t)(c.height - y - 1) * c.width + x);
+    float ax = film[fi], ay = film[fi + 1], az = film[fi + 2];
+    for (int s = 0; s < c.ns; ++s) {
+        const float4 L = P.L[(size_t)s * c.p_pad + q];
+        ax = fmaf(inv_spp, L.x * rcp_e(1.0f), ax);
+        ay = fmaf(inv_spp, L.y * rcp_e(1.0f), ay);
+        az = fmaf(inv_spp, L.z * rcp_e(1.0f), az);
+    }
+    film[fi] = ax; film[fi + 1] = ay; film[fi + 2] = az;
+}
+
+// NormalIntegrator.Li (Extensions/Integrator/Normal.fs:10-17) on the bins of
+// the first bounce: 0.5 * (n + 1).
+__global__ void k_shade_normal(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap) {
+    uint32_t start[kBins + 1];
+    start[0] = 0;
+    for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
+    const uint32_t total = start[kBins];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        int b = 0;
+        while (i >= start[b + 1]) ++b;
+        const uint32_t slot = bins[(size_t)b * bin_cap + (i - start[b])];
+        const Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
+        const int4 h = P.hit[slot];
+        const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+        P.L[slot] = f4(0.5f * (g.n + mk(1, 1, 1)), 0.0f);
+    }
+}
+
+// First hit of one sample per pixel: (instance, primitive, t) in pixel order.
+__global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int height, int sample_id, int spp,
+                                                   int frame_id, int* out_inst, int* out_prim, float* out_t,
+                                                   uint32_t* spill, size_t spill_stride) {
+    __shared__ uint32_t stk[kStackK * 256];
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    const size_t n = (size_t)width * height;
+    TraceCount tc{0, 0, 0};
+    for (size_t p = gid; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(p % width), y = (int)(p / width);
+        uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
+        V2 uPix, uLens;
+        uPix.x = lcg(rng); uPix.y = lcg(rng);
+        uLens.x = lcg(rng); uLens.y = lcg(rng);
+        const Ray r = camera_ray(S, x, y, uPix, uLens);
+        float t = kInf;
+        int hi = -1, hp = -1;
+        if (trace<false, false>(S, r, t, hi, hp, st, tc)) {
+            out_inst[p] = hi; out_prim[p] = hp; out_t[p] = t;
+        } else {
+            out_inst[p] = -1; out_prim[p] = -1; out_t[p] = kInf;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launch
+void launch_raygen(const DevScene& S, const PathBuf& P, Ctl* ctl, uint32_t* queue, const ChunkInfo& c,
+                   int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, st, S, P, ctl, queue, c);
+}
+void launch_reset(Ctl* ctl, int nxt, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt);
+}
+void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
+                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride,
+                          unsigned long long* stats, int grid, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
+                           spill, spill_stride, stats);
+    else
+        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
+                           spill, spill_stride, stats);
+}
+void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
+                  uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec, int depth, int max_depth,
+                  int rr_depth, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, S, P, ctl, bins, bin_cap, queue_next, nxt, shq, lrec,
+                       depth, max_depth, rr_depth);
+}
+void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
+                         uint32_t* spill, size_t spill_stride, unsigned long long* stats, int grid, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
+                           stats);
+    else
+        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
+                           stats);
+}
+void launch_film(const PathBuf& P, float* film, const ChunkInfo& c, float inv_spp, hipStream_t st) {
+    hipLaunchKernelGGL(k_film, dim3((c.width + 255) / 256, c.height), dim3(256), 0, st, P, film, c, inv_spp);
+}
+void launch_shade_normal(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
+                         int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade_normal, dim3(grid), dim3(256), 0, st, S, P, ctl, bins, bin_cap);
+}
+void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
+                      float* t, uint32_t* spill, size_t spill_stride, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_first_hit, dim3(grid), dim3(256), 0, st, S, w, h, sample_id, spp, frame_id, inst, prim, t,
+                       spill, spill_stride);
+}
+
+}  // namespace bnd

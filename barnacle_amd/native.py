@@ -1,0 +1,201 @@
+"""ctypes binding of libbarnacle_hip.so (the C ABI of include/barnacle_hip.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the library has not been built, and every call raises BarnacleError with
+the library's message on a non-zero status (the reference raises F#
+exceptions at the same points).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbarnacle_hip.so")
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+
+
+class BarnacleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_LIB = None
+
+_SIGS = {
+    "bn_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "bn_destroy": ([C.c_void_p], None),
+    "bn_last_error": ([], C.c_char_p),
+    "bn_abi_version": ([], C.c_int),
+    "bn_struct_sizes": ([i32p, C.c_int32], C.c_int),
+    "bn_bvh_build": ([f32p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p], C.c_int),
+    "bn_scene_build": ([C.c_void_p, C.POINTER(abi.SceneDesc), i32p], C.c_int),
+    "bn_scene_release": ([C.c_void_p, C.c_int32], C.c_int),
+    "bn_scene_export": ([C.c_void_p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p, i32p, f32p, f32p,
+                         i32p, i32p, f32p], C.c_int),
+    "bn_scene_export_blas": ([C.c_void_p, C.c_int32, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p, i32p,
+                              f32p], C.c_int),
+    "bn_upload_blas": ([C.c_void_p, C.POINTER(abi.BVHNode), C.c_int32, f32p, C.c_int32, i32p, C.c_int32,
+                        f32p, i32p], C.c_int),
+    "bn_upload_scene": ([C.c_void_p, C.POINTER(abi.BVHNode), C.c_int32, C.POINTER(abi.InstanceUpload),
+                         C.c_int32, C.POINTER(abi.Material), C.c_int32, C.POINTER(abi.Light), C.c_int32,
+                         C.POINTER(abi.Camera), f32p, i32p], C.c_int),
+    "bn_render_pt": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p, C.POINTER(abi.TraceStats)], C.c_int),
+    "bn_render_normal": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p], C.c_int),
+    "bn_first_hit": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p], C.c_int),
+    "bn_set_option": ([C.c_void_p, C.c_char_p, C.c_int64], C.c_int),
+    "bn_kernel_times": ([C.c_void_p, C.c_void_p, i32p], C.c_int),
+    "bn_reset_kernel_times": ([C.c_void_p], C.c_int),
+    "bn_render_pssmlt": ([C.c_void_p, C.c_int32, C.POINTER(abi.MLTParams), f32p, C.POINTER(abi.MLTStats)],
+                         C.c_int),
+}
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (idempotent).  Raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"libbarnacle_hip.so is not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise BarnacleError(rc, load().bn_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def bvh_build(bounds):
+    """BVHNode.Build (Util/BVH.fs:239-247) on the host: (nodes, perm)."""
+    lib = load()
+    b = np.ascontiguousarray(bounds, np.float32).reshape(-1, 6)
+    n = len(b)
+    nodes = (abi.BVHNode * (2 * n + 1))()
+    nn = C.c_int32(2 * n + 1)
+    perm = np.zeros(n, np.int32)
+    check(lib.bn_bvh_build(_p(b, f32p), n, nodes, C.byref(nn), _p(perm, i32p)))
+    from .bvh_util import nodes_to_numpy
+    return nodes_to_numpy(nodes, nn.value), perm
+
+
+class Context:
+    """One bn_ctx (device memory, stream) on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.bn_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.bn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, key: str, value: int):
+        check(self.lib.bn_set_option(self.h, key.encode(), int(value)))
+
+    def kernel_times(self):
+        """{kernel name: (launches, total ms)} accumulated since the last reset."""
+        buf = (abi.KernelTime * 64)()
+        n = C.c_int32(64)
+        check(self.lib.bn_kernel_times(self.h, buf, C.byref(n)))
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(n.value)}
+
+    def reset_kernel_times(self):
+        check(self.lib.bn_reset_kernel_times(self.h))
+
+    def scene(self, sc) -> "DeviceScene":
+        return DeviceScene(self, sc)
+
+
+class DeviceScene:
+    """A scene built on the host (BLAS/TLAS) and uploaded to the device."""
+
+    def __init__(self, ctx: Context, sc):
+        self.ctx, self.sc = ctx, sc
+        desc, keep = sc.desc()
+        sid = C.c_int32()
+        check(ctx.lib.bn_scene_build(ctx.h, C.byref(desc), C.byref(sid)))
+        self.id = sid.value
+
+    def render_pt(self, params: abi.PTParams, stats: bool = False):
+        """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters."""
+        film = np.zeros((params.width * params.height, 3), np.float32)
+        st = abi.TraceStats() if stats else None
+        check(self.ctx.lib.bn_render_pt(self.ctx.h, self.id, C.byref(params), _p(film, f32p),
+                                        C.byref(st) if st is not None else None))
+        if stats:
+            return film, {n: int(getattr(st, n)) for n, _ in abi.TraceStats._fields_}
+        return film
+
+    def render_normal(self, params: abi.PTParams):
+        film = np.zeros((params.width * params.height, 3), np.float32)
+        check(self.ctx.lib.bn_render_normal(self.ctx.h, self.id, C.byref(params), _p(film, f32p)))
+        return film
+
+    def first_hit(self, params: abi.PTParams, sample_id: int = 0):
+        n = params.width * params.height
+        inst, prim, t = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32)
+        check(self.ctx.lib.bn_first_hit(self.ctx.h, self.id, C.byref(params), sample_id, _p(inst, i32p),
+                                        _p(prim, i32p), _p(t, f32p)))
+        return inst, prim, t
+
+    def render_pssmlt(self, params: abi.MLTParams):
+        film = np.zeros((params.width * params.height, 3), np.float32)
+        st = abi.MLTStats()
+        check(self.ctx.lib.bn_render_pssmlt(self.ctx.h, self.id, C.byref(params), _p(film, f32p), C.byref(st)))
+        return film, {"accepted": st.accepted, "proposed": st.proposed, "b": np.float32(st.b),
+                      "mutations_per_chain": st.mutations_per_chain}
+
+    def export(self):
+        cap = 4 * max(1, len(self.sc.instances)) * 64 + 16
+        lib = self.ctx.lib
+        tl = (abi.BVHNode * cap)()
+        ntl, ni, nl = C.c_int32(cap), C.c_int32(cap), C.c_int32(cap)
+        order = np.zeros(cap, np.int32)
+        o2w, w2o = np.zeros((cap, 16), np.float32), np.zeros((cap, 16), np.float32)
+        lights = np.zeros(cap, np.int32)
+        c2w = np.zeros(16, np.float32)
+        check(lib.bn_scene_export(self.ctx.h, self.id, tl, C.byref(ntl), _p(order, i32p), C.byref(ni),
+                                  _p(o2w, f32p), _p(w2o, f32p), _p(lights, i32p), C.byref(nl), _p(c2w, f32p)))
+        from .bvh_util import nodes_to_numpy
+        n = ni.value
+        return {"tlas": nodes_to_numpy(tl, ntl.value), "inst_order": order[:n], "o2w": o2w[:n], "w2o": w2o[:n],
+                "lights": lights[:nl.value], "c2w": c2w}
+
+    def export_blas(self, prim: int):
+        p = self.sc.primitives[prim]
+        nt = p["n_indices"] // 3
+        nodes = (abi.BVHNode * (2 * nt + 1))()
+        nn, ntr = C.c_int32(2 * nt + 1), C.c_int32(nt)
+        tri = np.zeros(3 * nt, np.int32)
+        pdf = np.zeros(nt, np.float32)
+        check(self.ctx.lib.bn_scene_export_blas(self.ctx.h, self.id, prim, nodes, C.byref(nn), _p(tri, i32p),
+                                                C.byref(ntr), _p(pdf, f32p)))
+        from .bvh_util import nodes_to_numpy
+        return {"nodes": nodes_to_numpy(nodes, nn.value), "tri": tri, "alias_pdf": pdf}

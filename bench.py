@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: path-traced frames of the 1024x1024x64spp displaced-sphere
+Cornell box (69,620-triangle GGX mesh + box, depth 16) on one MI355X.
+
+A step is one full frame (PathTracingIntegrator.Render) with the scene
+already resident in HBM.  Prints one JSON line:
+  value          Mrays/s = (extension + shadow rays of one frame) * steps / time
+  roofline       dominant kernel (trace_closest): algorithmic bytes per launch
+                 (48 per ray + 32 per node + 36 per triangle + 48 per instance
+                 visited) / HIP-event launch duration, against 8 TB/s HBM
+  cpu_baseline   the CPU oracle (C++ restatement of the F# renderer) on a
+                 bounded pixel window of the same frame, all host threads up to 16
+  parity         GPU vs oracle on a 128x128x2spp render of the same scene
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from barnacle_amd import configs, native, scene  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def closest_bytes(st):
+    return 48 * st["rays_closest"] + 32 * st["nodes_closest"] + 36 * st["tris_closest"] + 48 * st["insts_closest"]
+
+
+def frame_bytes(st):
+    shadow = 36 * st["rays_shadow"] + 32 * st["nodes_shadow"] + 36 * st["tris_shadow"] + 48 * st["insts_shadow"]
+    return closest_bytes(st) + shadow + 192 * st["vertices"] + 12 * st["paths"]
+
+
+def cpu_baseline(sc, params, budget_s=12.0):
+    """Oracle rays/s on 64x64 pixel windows of the same frame, centre outwards."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    osc = oracle.Oracle().scene(sc)
+    W, H = params.width, params.height
+    tiles = sorted(((x, y) for y in range(0, H, 64) for x in range(0, W, 64)),
+                   key=lambda t: (t[0] + 32 - W / 2) ** 2 + (t[1] + 32 - H / 2) ** 2)
+    rays, paths, elapsed, n = 0, 0, 0.0, 0
+    for (x, y) in tiles:
+        t0 = time.perf_counter()
+        _, st = osc.render(params, threads=threads, window=(x, y, min(x + 64, W), min(y + 64, H)))
+        elapsed += time.perf_counter() - t0
+        rays += st["rays_closest"] + st["rays_shadow"]
+        paths += st["paths"]
+        n += 1
+        if elapsed >= budget_s:
+            break
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{n} 64x64-pixel windows at the film centre of the same 1024x1024x64spp frame "
+                      f"({paths} paths, {rays} rays, {elapsed:.1f} s); oracle = C++ restatement of the F# "
+                      f"renderer (no .NET runtime available)",
+            "mpaths_per_s": paths / elapsed / 1e6}
+
+
+def parity(ctx):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sc = scene.load_scene(configs.config1(width=128, height=128, spp=2))
+    p = sc.pt_params()
+    g = ctx.scene(sc).render_pt(p)
+    o, _ = oracle.Oracle().scene(sc).render(p, threads=min(16, os.cpu_count() or 1))
+    rel = float(np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)))
+    return {"config": "same scene at 128x128x2spp", "rel_rmse": rel, "bit_equal_fraction": float(np.mean(g == o)),
+            "tolerance": 1e-3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--spp", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    sc = scene.load_scene(configs.config1(spp=args.spp))
+    params = sc.pt_params()
+    ctx = native.Context(0)
+    ds = ctx.scene(sc)
+    print(f"# scene: {sc.n_triangles} triangles, {params.width}x{params.height}x{params.spp}spp depth "
+          f"{params.max_depth}", file=sys.stderr, flush=True)
+
+    # Counting pass (outside the timed region): exact per-frame work counts.
+    _, st = ds.render_pt(params, stats=True)
+    rays = st["rays_closest"] + st["rays_shadow"]
+    print(f"# counts: {json.dumps(st)}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        ds.render_pt(params)
+    ctx.set_option("timing", 1)
+    ctx.reset_kernel_times()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ds.render_pt(params)
+        print(f"# step {i} done", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.set_option("timing", 0)
+
+    ms_per_step = elapsed / args.steps * 1e3
+    launches, total_ms = kt["trace_closest"]
+    per_launch_ms = total_ms / launches
+    launches_per_frame = launches / args.steps
+    bytes_per_launch = closest_bytes(st) / launches_per_frame
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "trace_closest_config1_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "Mrays/s (extension + shadow rays), 1024x1024x64spp displaced-sphere Cornell box",
+        "value": rays * args.steps / elapsed / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "config1: Cornell box + 69,620-tri displaced icosphere (GGX), 1024x1024, "
+                               f"{params.spp} spp, max depth {params.max_depth}, rr depth {params.rr_depth}",
+                   "parallelism": "single GPU"},
+        "mpaths_per_s": params.width * params.height * params.spp * args.steps / elapsed / 1e6,
+        "rays_per_frame": {"extension": st["rays_closest"], "shadow": st["rays_shadow"]},
+        "frame_algorithmic_bytes": frame_bytes(st),
+        "frame_hbm_roofline_frac": frame_bytes(st) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"kernel": "trace_closest", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": per_launch_ms,
+                     "launches": launches},
+        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in kt.items()},
+    }
+    if not args.no_parity:
+        out["parity"] = parity(ctx)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, params)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,297 @@
+/*
+ * barnacle_hip.h -- C ABI of the MI355X wavefront path tracer.
+ *
+ * This is the drop-in boundary for the hot path of the Barnacle renderer
+ * (LeonKang130/Barnacle).  The reference's plugin seam is the F# abstract class
+ * IntegratorBase (Base/Integrator.fs:8-11), whose Render(camera, film,
+ * aggregate, lightSampler) is called once per frame by Scene.Render
+ * (Extensions/Scene/Render.fs:16).  A GPU integrator registered under a new
+ * "type" string in IntegratorInfo.ToIntegrator (Extensions/Scene/Loader.fs:185-204)
+ * binds the functions below through P/Invoke; the Python ctypes layer in
+ * barnacle_amd/abi.py binds exactly the same symbols.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative BN_E* code on failure;
+ *    bn_last_error() then holds a message (thread-local, valid until the next
+ *    call on that thread).  Nothing throws or aborts across the ABI; the
+ *    reference raises F# exceptions at the same places (failwith).
+ *  - Inputs are caller-owned host arrays borrowed for the duration of the call;
+ *    the library deep-copies what it keeps into device memory.  Outputs go to
+ *    caller-allocated host buffers.  All calls are blocking.
+ *  - One bn_ctx per host thread.  A context owns all device memory, streams and
+ *    graphs it creates.
+ *  - All structs are plain int32/float records (4-byte alignment, no padding).
+ */
+#ifndef BARNACLE_HIP_H
+#define BARNACLE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define BN_OK              0
+#define BN_E_INVALID      -1   /* bad argument / malformed scene            */
+#define BN_E_NO_LIGHT     -2   /* LightSamplerBase: no light primitives     */
+#define BN_E_DEVICE       -3   /* HIP runtime error                         */
+#define BN_E_UNSUPPORTED  -4   /* valid scene using a feature not built     */
+#define BN_E_CAPACITY     -5   /* size beyond a documented maximum          */
+
+/* ------------------------------------------------------ scene description
+ * A flat restatement of the reference's scene JSON schema
+ * (Extensions/Scene/Loader.fs:14-251).  The JSON front end
+ * (barnacle_amd/scene.py) resolves defaults exactly as Loader.fs does and
+ * expands the builtin "quad"/"cube" meshes (Extensions/Primitive/Mesh.fs:126-155).
+ */
+#define BN_PRIM_MESH    0
+#define BN_PRIM_SPHERE  1
+
+typedef struct bn_primitive {          /* PrimitiveInfo (Loader.fs:92-116)   */
+    int32_t kind;                      /* BN_PRIM_*                          */
+    int32_t vert_offset;               /* first vertex in vertices[] (xyz)   */
+    int32_t n_verts;
+    int32_t index_offset;              /* first index in indices[]           */
+    int32_t n_indices;                 /* 3 per triangle                     */
+    float   radius;                    /* sphere only                        */
+} bn_primitive;
+
+#define BN_MAT_LAMBERTIAN  0
+#define BN_MAT_MIRROR      1
+#define BN_MAT_DIELECTRIC  2
+#define BN_MAT_PBR         3
+
+typedef struct bn_material {           /* MaterialInfo (Loader.fs:50-73)     */
+    int32_t kind;                      /* BN_MAT_*                           */
+    float   albedo[3];
+    float   ior;
+    float   roughness;
+    float   metallic;
+} bn_material;
+
+typedef struct bn_light {              /* LightInfo (Loader.fs:75-90)        */
+    int32_t kind;                      /* 0 = diffuse (the only type)        */
+    float   emission[3];
+    int32_t two_sided;
+} bn_light;
+
+typedef struct bn_instance {           /* PrimitiveInstanceInfo (:118-145)   */
+    int32_t primitive;
+    int32_t material;                  /* -1 = none                          */
+    int32_t light;                     /* -1 = none                          */
+} bn_instance;
+
+#define BN_XF_SCALE        1
+#define BN_XF_ROTATION     2
+#define BN_XF_TRANSLATION  4
+
+typedef struct bn_transform {          /* the key frame Transform.Eval(0)    */
+    int32_t flags;                     /* BN_XF_* present components         */
+    float   scale[3];                  /* (Loader.fs:20-41): S*Rx*Ry*Rz*T    */
+    float   rotation[3];               /* radians                            */
+    float   translation[3];
+} bn_transform;
+
+typedef struct bn_node {               /* NodeInfo (Loader.fs:147-168)       */
+    int32_t transform;                 /* -1 = identity                      */
+    int32_t has_camera;
+    int32_t inst_offset;               /* into node_refs[]                   */
+    int32_t n_inst;
+    int32_t child_offset;              /* into node_refs[]                   */
+    int32_t n_child;
+} bn_node;
+
+#define BN_CAM_PINHOLE    0
+#define BN_CAM_THIN_LENS  1
+
+typedef struct bn_camera {             /* CameraInfo (Loader.fs:206-224)     */
+    int32_t kind;
+    float   fov;                       /* vertical, degrees                  */
+    float   aspect;                    /* from the camera, not the film      */
+    float   aperture;
+    float   focus_distance;
+    float   push_forward;
+} bn_camera;
+
+typedef struct bn_scene_desc {
+    const float*        vertices;     int64_t n_vertex_floats;
+    const int32_t*      indices;      int64_t n_indices;
+    const bn_primitive* primitives;   int32_t n_primitives;
+    const bn_material*  materials;    int32_t n_materials;
+    const bn_light*     lights;       int32_t n_lights;
+    const bn_instance*  instances;    int32_t n_instances;
+    const bn_transform* transforms;   int32_t n_transforms;
+    const bn_node*      nodes;        int32_t n_nodes;
+    const int32_t*      node_refs;    int32_t n_node_refs;
+    int32_t             root;
+    bn_camera           camera;
+} bn_scene_desc;
+
+/* ------------------------------------------------------ render parameters */
+typedef struct bn_pt_params {          /* PathTracingIntegrator + Film       */
+    int32_t width, height;             /* FilmInfo (Loader.fs:226-239)       */
+    int32_t spp;                       /* "spp"       default 1              */
+    int32_t max_depth;                 /* "max-depth" default 8              */
+    int32_t rr_depth;                  /* "rr-depth"  default 5              */
+    int32_t frame_id;                  /* ProgressiveIntegrator.FrameId      */
+} bn_pt_params;
+
+#define BN_MLT_GAUSSIAN  0
+#define BN_MLT_KELEMEN   1
+
+typedef struct bn_mlt_params {         /* PSSMLTIntegrator (PSSMLT.fs:153)   */
+    int32_t width, height;
+    int32_t spp;                       /* mutations per pixel                */
+    int32_t max_depth, rr_depth, frame_id;
+    int32_t n_bootstrap;               /* default 4*1024*1024                */
+    int32_t n_chains;                  /* default 1024                       */
+    int32_t strategy;                  /* BN_MLT_*                           */
+    float   sigma;                     /* Gaussian sigma (reference: 1e-2)   */
+    float   eps_min, eps_max;          /* Kelemen (reference: 1/1024, 1/16)  */
+    float   large_step_prob;           /* default 0.5                        */
+} bn_mlt_params;
+
+typedef struct bn_mlt_stats {          /* PSSMLT.fs:412-414 prints these     */
+    int64_t accepted;
+    int64_t proposed;
+    float   b;                         /* mean bootstrap luminance           */
+    int32_t mutations_per_chain;
+} bn_mlt_stats;
+
+/* Per-launch counters of the traversal, for the algorithmic-bytes roofline. */
+typedef struct bn_trace_stats {
+    int64_t rays_closest, rays_shadow;
+    int64_t nodes_closest, tris_closest, insts_closest;
+    int64_t nodes_shadow,  tris_shadow,  insts_shadow;
+    int64_t vertices;                  /* path vertices shaded               */
+    int64_t paths;
+} bn_trace_stats;
+
+/* Reference BVHNode, 32 bytes, explicit layout of Util/BVH.fs:52-73. */
+typedef struct bn_bvh_node {
+    float   pmin[3];
+    float   pmax[3];
+    int32_t right_or_offset;           /* RightChild | InstanceOffset @24    */
+    uint8_t is_leaf;                   /* @28                                */
+    int8_t  split_axis;                /* @29                                */
+    int8_t  count;                     /* @30                                */
+    uint8_t mask;                      /* @31 VisibilityMask (unused)        */
+} bn_bvh_node;
+
+/* ----------------------------------------------------------- the context */
+typedef struct bn_ctx bn_ctx;
+
+/* No reference counterpart (the reference has no device); the F# integrator
+ * creates one context in its constructor. */
+int         bn_create(int device, bn_ctx** out);
+void        bn_destroy(bn_ctx* ctx);
+const char* bn_last_error(void);
+int         bn_abi_version(void);
+/* sizeof of every struct above, in declaration order, for binding checks. */
+int         bn_struct_sizes(int32_t* out, int32_t n);
+
+/* BVHNode.Build (Util/BVH.fs:239-247): binned-SAH build over item bounds
+ * (6 floats per item: pmin xyz, pmax xyz).  Writes up to 2*n nodes in the
+ * reference's pre-order and the permutation perm[i] = original index of the
+ * item now at position i.  Host only; the F# host calls its own BVHNode.Build
+ * and uses bn_upload_* instead. */
+int bn_bvh_build(const float* item_bounds, int32_t n_items,
+                 bn_bvh_node* out_nodes, int32_t* out_n_nodes, int32_t* out_perm);
+
+/* Scene construction as Scene.Load + Scene.Render(0) perform it before the
+ * timed region (Loader.fs:253-272, Base/Scene.fs:38-51, Render.fs:11-14):
+ * every mesh BLAS is built (Mesh.fs:159-169), the node graph is traversed DFS
+ * with world = local * parent, the TLAS is built over the instance array
+ * (Extensions/Aggregate/BVH.fs:9) and the light list is taken in TLAS order
+ * (Base/LightSampler.fs:7).  Everything is uploaded to the device. */
+int bn_scene_build(bn_ctx* ctx, const bn_scene_desc* desc, int32_t* out_scene);
+int bn_scene_release(bn_ctx* ctx, int32_t scene);
+
+/* Read back what bn_scene_build derived, for parity checks against an
+ * independent build.  n_* are in/out: capacity in, count out. */
+int bn_scene_export(bn_ctx* ctx, int32_t scene,
+                    bn_bvh_node* tlas, int32_t* n_tlas,
+                    int32_t* inst_order, int32_t* n_inst,
+                    float* inst_o2w /* 16 per instance */,
+                    float* inst_w2o /* 16 per instance */,
+                    int32_t* light_insts, int32_t* n_lights,
+                    float* camera_to_world /* 16 */);
+int bn_scene_export_blas(bn_ctx* ctx, int32_t scene, int32_t primitive,
+                         bn_bvh_node* nodes, int32_t* n_nodes,
+                         int32_t* tri_indices /* 3 per tri, BVH order */, int32_t* n_tris,
+                         float* alias_pdf /* per tri */);
+
+/* Pre-built upload path for the F# host, which builds the BVHs itself with
+ * the reference code.  BLAS: MeshPrimitive.BVHNodes / TriangleIndices /
+ * Vertices / AliasTable pdfs (Mesh.fs:157-186).  Scene: the TLAS nodes and the
+ * permuted instance array of BVHAggregate (Extensions/Aggregate/BVH.fs:9) with
+ * ObjectToWorld / WorldToObject / Bounds (Base/Primitive.fs:103-105) as
+ * row-major 4x4 (System.Numerics layout: translation in M41..M43). */
+typedef struct bn_instance_upload {
+    float   o2w[16];
+    float   w2o[16];
+    float   bounds[6];                 /* world AABB pmin xyz, pmax xyz      */
+    int32_t blas;                      /* id from bn_upload_blas, or -1      */
+    float   sphere_radius;             /* used when blas == -1               */
+    int32_t material;                  /* index into materials, -1 = none    */
+    int32_t light;                     /* index into lights,    -1 = none    */
+} bn_instance_upload;
+
+int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes,
+                   const float* vertices, int32_t n_verts,
+                   const int32_t* tri_indices, int32_t n_tris,
+                   const float* alias_pdf, int32_t* out_blas);
+int bn_upload_scene(bn_ctx* ctx, const bn_bvh_node* tlas, int32_t n_tlas,
+                    const bn_instance_upload* instances, int32_t n_instances,
+                    const bn_material* materials, int32_t n_materials,
+                    const bn_light* lights, int32_t n_lights,
+                    const bn_camera* camera, const float* camera_to_world,
+                    int32_t* out_scene);
+
+/* PathTracingIntegrator.Render (Extensions/Integrator/PathTracing.fs:14-81
+ * through ProgressiveIntegrator.RenderTile/Render, Base/Integrator.fs:22-55):
+ * writes film_rgb[W*H*3] in Film.Pixels order, row (H-y-1) first
+ * (Base/Film.fs:41-46).  stats may be NULL; when given, a counting pass of
+ * the traversal fills it (that pass is slower and not what is timed). */
+int bn_render_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
+                 float* film_rgb, bn_trace_stats* stats);
+
+/* NormalIntegrator (Extensions/Integrator/Normal.fs:10-17), the first-hit view. */
+int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
+                     float* film_rgb);
+
+/* First hit of sample `sample_id` of every pixel: the closest-hit query of
+ * BVHAggregate.Intersect (Extensions/Aggregate/BVH.fs:37-58).  Outputs are in
+ * pixel order y*W+x: TLAS-order instance index (-1 = miss), primitive index
+ * in BVH order (the Mesh.fs:232 tag; sphere: 0 near root, 1 far root) and t. */
+int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
+                 int32_t sample_id, int32_t* inst, int32_t* prim, float* t);
+
+/* Tunables and instrumentation (no reference counterpart).  Keys:
+ *   "timing"               1: record a HIP event pair around every kernel
+ *                          launch of the render calls, on the context's stream
+ *   "chunk_slots"          path slots in flight per bounce loop (default 8M)
+ *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels */
+int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
+
+typedef struct bn_kernel_time {
+    char    name[32];
+    int64_t launches;
+    double  total_ms;                  /* sum of event-measured durations    */
+} bn_kernel_time;
+/* Per-kernel totals accumulated since the last reset (n: capacity in,
+ * count out). */
+int bn_kernel_times(bn_ctx* ctx, bn_kernel_time* out, int32_t* n);
+int bn_reset_kernel_times(bn_ctx* ctx);
+
+/* PSSMLTIntegrator.Render (Extensions/Integrator/PSSMLT.fs:379-414).  The film
+ * is cleared first (Film.Clear, Render.fs:11) and accumulated with device
+ * atomics (the reference's Film.Accumulate is an unsynchronised add). */
+int bn_render_pssmlt(bn_ctx* ctx, int32_t scene, const bn_mlt_params* params,
+                     float* film_rgb, bn_mlt_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BARNACLE_HIP_H */

@@ -1,0 +1,71 @@
+"""GPU (HIP, through the C ABI) against the CPU oracle on the same scenes.
+
+Bars: first-hit (instance, primitive, t) bit-exact; films bit-exact except
+for rare transcendental rounding differences, and within rel-RMSE 1e-3 in any
+case (the tolerance stated for the float path).
+"""
+import numpy as np
+import pytest
+
+from barnacle_amd import configs, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rmse(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return scene.load_scene(configs.config0())
+
+
+@pytest.fixture(scope="module")
+def c1_small():
+    return scene.load_scene(configs.config1(width=128, height=128, spp=2))
+
+
+def test_first_hit_config0(ctx, orc, c0):
+    p = c0.pt_params()
+    gi, gp, gt = ctx.scene(c0).first_hit(p, 0)
+    oi, op, ot = orc.scene(c0).first_hit(p, 0)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+
+
+def test_first_hit_config1(ctx, orc, c1_small):
+    p = c1_small.pt_params(width=256, height=256)
+    for k in (0, 1):
+        gi, gp, gt = ctx.scene(c1_small).first_hit(p, k)
+        oi, op, ot = orc.scene(c1_small).first_hit(p, k)
+        assert np.array_equal(gi, oi)
+        assert np.array_equal(gp, op)
+        assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+
+
+def test_pt_config0_film(ctx, orc, c0):
+    p = c0.pt_params()
+    g = ctx.scene(c0).render_pt(p)
+    o, _ = orc.scene(c0).render(p)
+    assert np.all(np.isfinite(g))
+    assert rel_rmse(g, o) <= 1e-3
+    assert np.mean(g == o) > 0.999
+
+
+def test_pt_config1_film(ctx, orc, c1_small):
+    p = c1_small.pt_params()
+    g = ctx.scene(c1_small).render_pt(p)
+    o, _ = orc.scene(c1_small).render(p)
+    assert rel_rmse(g, o) <= 1e-3
+    assert np.mean(g == o) > 0.999
+
+
+def test_trace_counters_match_oracle(ctx, orc, c0):
+    p = c0.pt_params(width=64, height=64, spp=2)
+    _, gs = ctx.scene(c0).render_pt(p, stats=True)
+    _, os_ = orc.scene(c0).render(p)
+    for k in ("rays_closest", "rays_shadow", "nodes_closest", "tris_closest", "insts_closest",
+              "nodes_shadow", "tris_shadow", "insts_shadow", "vertices", "paths"):
+        assert gs[k] == os_[k], k

@@ -1,0 +1,137 @@
+"""Oracle materials (Extensions/Material/*.fs) against closed forms.
+
+Each check is chosen so that a plausible transcription error fails it:
+Lambertian cosine sampling (distribution of cos theta, f/pdf = albedo, the
+hemisphere flip), mirror reflection, Schlick-Fresnel at normal incidence,
+Snell's law and total internal reflection, the GGX half-vector sampling
+distribution (closed-form CDF of theta_h) and the normalisation of the
+specular pdf (integral over the sphere = 1, which fails for a dropped 4 or a
+wrong Jacobian)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from barnacle_amd import abi
+
+
+def mat(kind, albedo=(0.5, 0.6, 0.7), ior=1.5, roughness=1.0, metallic=0.0):
+    return abi.Material(kind, (C.c_float * 3)(*albedo), ior, roughness, metallic)
+
+
+def unit(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
+
+
+def test_lambert_cosine_sampling_distribution(orc):
+    m = mat(0)
+    wo = unit([0.3, -0.2, 0.9])
+    rng = np.random.RandomState(1)
+    cos = []
+    for u in rng.uniform(0, 1, (4000, 2)).astype(np.float32):
+        s = orc.material_sample(m, wo, 0.5, u)
+        wi = s[4:]
+        assert wi[2] >= 0
+        assert s[3] == pytest.approx(wi[2] / np.pi, rel=1e-6)
+        assert np.allclose(s[:3] / s[3], [0.5, 0.6, 0.7], rtol=1e-6)
+        assert np.linalg.norm(wi) == pytest.approx(1.0, abs=1e-6)
+        cos.append(wi[2])
+    # P(cos <= c) = c^2  (pdf cos/pi over the hemisphere)
+    cos = np.sort(cos)
+    ks = np.max(np.abs(np.arange(1, len(cos) + 1) / len(cos) - cos ** 2))
+    assert ks < 1.63 / np.sqrt(len(cos))      # KS at alpha = 1%
+
+
+def test_lambert_flips_to_wo_side_and_eval(orc):
+    m = mat(0)
+    s = orc.material_sample(m, unit([0.1, 0.2, -0.97]), 0.5, [0.3, 0.4])
+    assert s[6] < 0
+    e = orc.material_eval(m, unit([0, 0, 1]), unit([0.6, 0, 0.8]))
+    assert e[3] == pytest.approx(0.8 / np.pi, rel=1e-6)
+    assert np.allclose(e[:3], np.array([0.5, 0.6, 0.7]) * 0.8 / np.pi, rtol=1e-6)
+    assert np.all(orc.material_eval(m, unit([0, 0, 1]), unit([0.6, 0, -0.8])) == 0)
+
+
+def test_mirror(orc):
+    m = mat(1)
+    wo = unit([0.3, -0.4, 0.5])
+    s = orc.material_sample(m, wo, 0.1, [0.2, 0.3])
+    assert np.array_equal(s[4:], np.array([-wo[0], -wo[1], wo[2]], np.float32))
+    assert s[3] == 1.0 and np.allclose(s[:3], [0.5, 0.6, 0.7])
+    assert np.all(orc.material_eval(m, wo, s[4:]) == 0)
+
+
+def test_dielectric_normal_incidence_schlick(orc):
+    m = mat(2, albedo=(1, 1, 1), ior=1.5)
+    wo = np.array([0, 0, 1], np.float32)
+    r0 = ((1 / 1.5 - 1) / (1 / 1.5 + 1)) ** 2          # = 0.04
+    refl = orc.material_sample(m, wo, 0.01, [0.5, 0.5])
+    assert refl[3] == pytest.approx(r0, rel=1e-5) and refl[6] == 1.0
+    refr = orc.material_sample(m, wo, 0.5, [0.5, 0.5])
+    assert refr[3] == pytest.approx(1 - r0, rel=1e-6)
+    assert np.allclose(refr[4:], [0, 0, -1])
+
+
+def test_dielectric_snell_and_tir(orc):
+    m = mat(2, albedo=(1, 1, 1), ior=1.5)
+    for th in np.radians([10, 35, 60, 80]):
+        wo = np.array([np.sin(th), 0, np.cos(th)], np.float32)
+        s = orc.material_sample(m, wo, 0.999, [0.5, 0.5])
+        wi = s[4:]
+        # from outside: sin(theta_t) = sin(theta_i) / 1.5, transmitted below
+        assert wi[2] < 0
+        assert np.hypot(wi[0], wi[1]) == pytest.approx(np.sin(th) / 1.5, rel=1e-5)
+        assert np.linalg.norm(wi) == pytest.approx(1.0, abs=1e-5)
+    # from inside beyond the critical angle asin(1/1.5) = 41.8 deg: reflection, pdf 1
+    th = np.radians(60)
+    wo = np.array([np.sin(th), 0, -np.cos(th)], np.float32)
+    s = orc.material_sample(m, wo, 0.999, [0.5, 0.5])
+    assert s[3] == 1.0 and np.allclose(s[4:], [-wo[0], -wo[1], wo[2]])
+
+
+@pytest.mark.parametrize("roughness", [0.3, 0.7])
+def test_ggx_half_vector_distribution(orc, roughness):
+    """theta_h = atan(alpha sqrt(u/(1-u))) has CDF tan^2/(alpha^2 + tan^2)."""
+    alpha = max(roughness * roughness, 1e-3)
+    m = mat(3, metallic=1.0, roughness=roughness)
+    wo = unit([0.2, 0.1, 0.97])
+    rng = np.random.RandomState(5)
+    th = []
+    for u in rng.uniform(0, 1, (3000, 2)).astype(np.float32):
+        s = orc.material_sample(m, wo, 0.0, u)
+        wh = unit(wo.astype(np.float64) + s[4:].astype(np.float64))
+        th.append(np.arccos(np.clip(wh[2], -1, 1)))
+    th = np.sort(np.array(th))
+    t2 = np.tan(th) ** 2
+    cdf = t2 / (alpha ** 2 + t2)
+    ks = np.max(np.abs(np.arange(1, len(th) + 1) / len(th) - cdf))
+    assert ks < 1.63 / np.sqrt(len(th)) + 2e-3
+
+
+def test_ggx_specular_pdf_integrates_to_one(orc):
+    """For metallic = 1 the pdf is D |wh.z| / (4 wo.wh); over all wi it
+    integrates to the visible GGX mass, 1 at normal incidence."""
+    m = mat(3, metallic=1.0, roughness=0.6)
+    wo = np.array([0, 0, 1], np.float32)
+    n_t, n_p = 400, 64
+    tq = (np.arange(n_t) + 0.5) / n_t * np.pi
+    total = 0.0
+    for t in tq:
+        # pdf is azimuthally symmetric for wo = z; integrate one azimuth
+        wi = np.array([np.sin(t), 0, np.cos(t)], np.float32)
+        pdf = orc.material_eval(m, wo, wi)[3]
+        if np.isfinite(pdf):
+            total += pdf * np.sin(t) * (np.pi / n_t) * 2 * np.pi
+    assert total == pytest.approx(1.0, abs=2e-2)
+    del n_p
+
+
+def test_pbr_diffuse_lobe_pdf_mix(orc):
+    """metallic = 0: pdf = mix(spec pdf, cos/pi, 1/2) (PBR.fs:123)."""
+    m = mat(3, metallic=0.0, roughness=1.0)
+    wo = np.array([0, 0, 1], np.float32)
+    wi = unit([0.3, 0.2, 0.9])
+    e0 = orc.material_eval(mat(3, metallic=1.0, roughness=1.0), wo, wi)
+    e = orc.material_eval(m, wo, wi)
+    assert e[3] == pytest.approx(0.5 * e0[3] + 0.5 * wi[2] / np.pi, rel=1e-5)
