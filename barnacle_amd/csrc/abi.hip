@@ -240,6 +240,17 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
     sc.dev.alias_pdf = (const float*)sc.apdf.p;
     sc.dev.c2w = to_xf(c2w);
     sc.dev.cam_kind = cam.kind;
+    bool finite = true;
+    auto fin_nodes = [&](const std::vector<bn_bvh_node>& v) {
+        for (const auto& n : v)
+            for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(n.pmin[k]) && std::isfinite(n.pmax[k]);
+    };
+    fin_nodes(tlas);
+    fin_nodes(nodes);
+    for (float v : tris) finite = finite && std::isfinite(v);
+    for (const auto& in : insts)
+        for (int k = 0; k < 6; ++k) finite = finite && std::isfinite(in.bounds[k]);
+    sc.dev.exact_only = finite ? 0 : 1;
     sc.live = true;
 }
 
@@ -338,7 +349,9 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
             { Timed tm(T, "trace_closest", s);
               bnd::launch_trace_closest(count, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
                                         (uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->spill.p, c->spill_stride,
-                                        (unsigned long long*)c->stats.p, gt, s); }
+                                        (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
+            { Timed tm(T, "bin", s);
+              bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
             if (normal) {
                 Timed tm(T, "shade_normal", s);
                 bnd::launch_shade_normal(sc.dev, P, ctl, (const uint32_t*)c->bins.p, bin_cap, gs, s);
@@ -350,7 +363,7 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
                                 gs, s); }
             { Timed tm(T, "trace_shadow", s);
               bnd::launch_trace_shadow(count, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
-                                       c->spill_stride, (unsigned long long*)c->stats.p, gt, s); }
+                                       c->spill_stride, (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
             if (count) {
                 bnd::Ctl h;
                 hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "ctl readback");
@@ -626,9 +639,11 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* p, int32_t samp
         c->out_a.ensure(n * 4);
         c->out_b.ensure(n * 4);
         c->out_c.ensure(n * 4);
+        c->ctl.ensure(sizeof(bnd::Ctl));
+        hip_check(hipMemsetAsync(c->ctl.p, 0, sizeof(bnd::Ctl), c->stream), "memset ctl");
         bnd::launch_first_hit(s.dev, p->width, p->height, sample_id, p->spp, p->frame_id, (int*)c->out_a.p,
                               (int*)c->out_b.p, (float*)c->out_c.p, (uint32_t*)c->spill.p, c->spill_stride,
-                              grid_trace(c), c->stream);
+                              &((bnd::Ctl*)c->ctl.p)->fetch[0], grid_trace(c), c->stream);
         hip_check(hipGetLastError(), "kernel launch");
         hip_check(hipMemcpyAsync(inst, c->out_a.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipMemcpyAsync(prim, c->out_b.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
@@ -659,6 +674,9 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "trace_blocks_per_cu") {
             if (value < 1 || value > 5) throw Error(BN_E_INVALID, "trace_blocks_per_cu must be in [1, 5]");
             c->trace_blocks_per_cu = (int)value;
+        } else if (k == "refill") {
+            if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
+            c->refill = (int)value;
         } else throw Error(BN_E_INVALID, "unknown option " + k);
     });
 }

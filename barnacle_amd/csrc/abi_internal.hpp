@@ -64,6 +64,7 @@ struct Ctx {
     int n_cu = 1;
     hipStream_t stream = nullptr;
     int trace_blocks_per_cu = 4;
+    int refill = 48;                      // idle lanes that trigger a refill
     size_t chunk_slots = (size_t)8 << 20;
     std::vector<BlasRec> blas;
     std::vector<std::unique_ptr<SceneRec>> scenes;

@@ -136,6 +136,7 @@ struct DevScene {
     float4 du;                  // pixel delta U, w = aperture
     float4 dv;                  // pixel delta V, w = focus distance
     int cam_kind;
+    int exact_only;             // scene has non-finite geometry: exact slab tests only
 };
 
 struct Ray { V3 o, d; };
@@ -233,97 +234,5 @@ struct LaneStack {
 };
 
 struct TraceCount { uint32_t nodes, tris, insts; };
-
-// BVHAggregate.Intersect (Extensions/Aggregate/BVH.fs:11-58) over
-// PrimitiveInstance.Intersect (Base/Primitive.fs:111-129) and
-// MeshPrimitive.Intersect (Extensions/Primitive/Mesh.fs:188-242), in the
-// reference's visit order: pop, slab-test with the current t, leaf -> items
-// in order, interior -> push far then near (near = left when dir[axis] > 0).
-// ANY: returns at the first hit with t fixed; closest: t shrinks, and
-// (inst, prim) name the last accepted hit.
-template <bool ANY, bool COUNT, int K>
-__device__ bool trace(const DevScene& S, const Ray& r, float& t, int& hit_inst, int& hit_prim,
-                      LaneStack<K>& st, TraceCount& cnt) {
-    const V3 inv = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-    st.sp = 0;
-    st.push(0);
-    bool hit = false;
-    int leaf_cur = 0, leaf_end = 0;
-    int blas_base = -1;                 // stack depth at BLAS entry, -1 = in TLAS
-    int cur_inst = 0, tri_base = 0, blas_root = 0;
-    V3 oo = {0, 0, 0}, od = {0, 0, 0}, oinv = {0, 0, 0};
-    for (;;) {
-        if (blas_base >= 0) {
-            if (st.sp == blas_base) { blas_base = -1; continue; }
-            const int ni = (int)st.pop();
-            const NodeView n = load_node(S.blas, blas_root + ni);
-            if (COUNT) cnt.nodes++;
-            if (!slab(n.lo, n.hi, oo, oinv, t)) continue;
-            if (n.leaf) {
-                for (int k = n.word; k < n.word + n.count; ++k) {
-                    const Tri tr = load_tri(S.tris, tri_base + k);
-                    if (COUNT) cnt.tris++;
-                    const V3 lo = vmin(vmin(tr.p0, tr.p1), tr.p2), hi = vmax(vmax(tr.p0, tr.p1), tr.p2);
-                    if (!slab(lo, hi, oo, oinv, t)) continue;
-                    float tt;
-                    if (tri_hit(tr, oo, od, t, tt)) {
-                        hit = true;
-                        if (ANY) return true;
-                        t = tt;
-                        hit_inst = cur_inst;
-                        hit_prim = k;
-                    }
-                }
-            } else if (comp(od, n.axis) > 0.0f) {
-                st.push((uint32_t)n.word); st.push((uint32_t)(ni + 1));
-            } else {
-                st.push((uint32_t)(ni + 1)); st.push((uint32_t)n.word);
-            }
-        } else if (leaf_cur < leaf_end) {
-            const int ii = leaf_cur++;
-            const DevInst& I = S.insts[ii];
-            const float4 lo = I.lo, hi = I.hi;
-            if (!slab(xyz(lo), xyz(hi), r.o, inv, t)) continue;
-            if (COUNT) cnt.insts++;
-            const Xf w2o = I.w2o;
-            oo = xf_point(w2o, r.o);
-            od = xf_dir(w2o, r.d);
-            const int4 info = I.info;
-            if (info.x < 0) {
-                float tt;
-                int br;
-                if (sphere_hit(lo.w, oo, od, t, tt, br)) {
-                    hit = true;
-                    if (ANY) return true;
-                    t = tt;
-                    hit_inst = ii;
-                    hit_prim = br;
-                }
-                continue;
-            }
-            oinv = {1.0f / od.x, 1.0f / od.y, 1.0f / od.z};
-            cur_inst = ii;
-            tri_base = info.y;
-            blas_root = info.x;
-            blas_base = st.sp;
-            st.push(0);
-        } else {
-            if (st.sp == 0) break;
-            const int ni = (int)st.pop();
-            const NodeView n = load_node(S.tlas, ni);
-            if (COUNT) cnt.nodes++;
-            if (!slab(n.lo, n.hi, r.o, inv, t)) continue;
-            if (n.leaf) {
-                leaf_cur = n.word;
-                leaf_end = n.word + n.count;
-            } else if (comp(r.d, n.axis) > 0.0f) {
-                st.push((uint32_t)n.word); st.push((uint32_t)(ni + 1));
-            } else {
-                st.push((uint32_t)(ni + 1)); st.push((uint32_t)n.word);
-            }
-        }
-    }
-    return hit;
-}
 
 }  // namespace bnd

@@ -47,17 +47,20 @@ void launch_raygen(const DevScene& S, const PathBuf& P, Ctl* ctl, uint32_t* queu
                    int grid, hipStream_t st);
 void launch_reset(Ctl* ctl, int nxt, hipStream_t st);
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride,
+                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride, uint32_t refill,
                           unsigned long long* stats, int grid, hipStream_t st);
+void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
+                int grid, hipStream_t st);
 void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
                   uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec, int depth, int max_depth,
                   int rr_depth, int grid, hipStream_t st);
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         uint32_t* spill, size_t spill_stride, unsigned long long* stats, int grid, hipStream_t st);
+                         uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
+                         hipStream_t st);
 void launch_film(const PathBuf& P, float* film, const ChunkInfo& c, float inv_spp, hipStream_t st);
 void launch_shade_normal(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
                          int grid, hipStream_t st);
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
-                      float* t, uint32_t* spill, size_t spill_stride, int grid, hipStream_t st);
+                      float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st);
 
 }  // namespace bnd

@@ -15,6 +15,7 @@
 // (Base/Integrator.fs:36-44).
 #include "kernels.hpp"
 #include "shading.hpp"
+#include "traverse.hpp"
 
 namespace bnd {
 
@@ -90,42 +91,120 @@ __global__ void k_reset(Ctl* ctl, int nxt) {
 }
 
 // ------------------------------------------------------------ trace closest
-// BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  Hits
-// are appended to the bin of their instance's material kind (light-only
-// instances to the last bin) so that shade runs over coherent waves.
+// Lane-refilling persistent traversal: a wave keeps up to 64 rays in flight,
+// advances each by one item per iteration and refills idle lanes from the
+// queue once at least `refill` of them are idle (one atomic per wave).
+template <bool ANY, bool COUNT, typename Init, typename Done>
+__device__ __forceinline__ void persistent_trace(const DevScene& S, uint32_t* fetch_ctr, uint32_t count,
+                                                 uint32_t* stk, uint32_t* spill, size_t spill_stride,
+                                                 uint32_t refill, TraceCount& tc, Init init, Done done) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Traversal<kStackK> tr;
+    tr.st = LaneStack<kStackK>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    bool active = false, more = true;
+    uint32_t item = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && n_idle >= refill) {
+            const uint32_t base = wave_fetch(fetch_ctr, n_idle);
+            if (base + n_idle >= count) more = false;
+            if (!active) {
+                const uint32_t i = base + mbcnt(idle);
+                if (i < count) {
+                    item = i;
+                    init(tr, i);
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (!more) break;
+            continue;
+        }
+        bool fin = false, found = false;
+        if (active) fin = !tr.template step<ANY, COUNT>(S, tc, found);
+        if (__ballot(fin) != 0) {
+            if (fin) active = false;
+            done(tr, item, fin, found);
+        }
+    }
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_closest(DevScene S, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                                                       uint32_t* spill, size_t spill_stride,
+                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                        unsigned long long* stats) {
     __shared__ uint32_t stk[kStackK * 256];
     const uint32_t count = ctl->q_count[cur];
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
     TraceCount tc{0, 0, 0};
-    for (;;) {
-        const uint32_t base = wave_fetch(&ctl->fetch[0], 64);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        int bin = -1;
-        uint32_t slot = 0;
-        if (i < count) {
-            slot = queue[i];
-            Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
-            float t = kInf;
-            int hi = -1, hp = -1;
-            if (trace<false, COUNT>(S, r, t, hi, hp, st, tc)) {
-                P.hit[slot] = make_int4(hi, hp, __float_as_int(t), 0);
-                const int4 info = S.insts[hi].info;
-                bin = info.z >= 0 ? __float_as_int(S.mats[2 * info.z].x) : kBins - 1;
+    persistent_trace<false, COUNT>(
+        S, &ctl->fetch[0], count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<kStackK>& tr, uint32_t i) {
+            const uint32_t slot = queue[i];
+            tr.start({xyz(P.o[slot]), xyz(P.d[slot])}, kInf, S.exact_only != 0);
+        },
+        [&](Traversal<kStackK>& tr, uint32_t i, bool fin, bool) {
+            // the material bin travels with the hit; k_bin compacts the bins
+            if (fin) {
+                const uint32_t slot = queue[i];
+                int bin = -1;
+                if (tr.hit_inst >= 0) {
+                    const int4 info = S.insts[tr.hit_inst].info;
+                    bin = info.z >= 0 ? __float_as_int(S.mats[2 * info.z].x) : kBins - 1;
+                }
+                P.hit[slot] = make_int4(tr.hit_inst, tr.hit_prim, __float_as_int(tr.t), bin);
             }
-        }
-        for (int b = 0; b < kBins; ++b) wave_push(bin == b, &ctl->bin_count[b], bins + (size_t)b * bin_cap, slot);
-    }
+        });
     if (COUNT) {
         wave_add_stats(&stats[0], tc.nodes);
         wave_add_stats(&stats[1], tc.tris);
         wave_add_stats(&stats[2], tc.insts);
+    }
+}
+
+// Material binning of the closest hits (sorting the shade queue by material
+// kind).  A block bins kBinItems consecutive queue entries: positions inside
+// the block come from wave-aggregated LDS counters, and each bin costs one
+// global atomic per block.
+constexpr int kBinPer = 16;                       // entries per thread
+constexpr int kBinItems = 256 * kBinPer;
+
+__global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins,
+                                             uint32_t bin_cap) {
+    __shared__ uint32_t cnt[kBins], gbase[kBins];
+    const uint32_t count = ctl->q_count[cur];
+    for (uint32_t blk = blockIdx.x * (uint32_t)kBinItems; blk < count; blk += gridDim.x * (uint32_t)kBinItems) {
+        if (threadIdx.x < kBins) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t slot[kBinPer], pos[kBinPer];
+        int bin[kBinPer];
+        for (int k = 0; k < kBinPer; ++k) {
+            const uint32_t i = blk + (uint32_t)k * 256u + threadIdx.x;
+            bin[k] = -1;
+            slot[k] = 0;
+            if (i < count) {
+                slot[k] = queue[i];
+                bin[k] = P.hit[slot[k]].w;
+            }
+            pos[k] = 0;
+            for (int b = 0; b < kBins; ++b) {
+                const uint64_t m = __ballot(bin[k] == b);
+                if (m == 0) continue;
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&cnt[b], (uint32_t)__popcll(m));
+                base = __shfl(base, leader);
+                if (bin[k] == b) pos[k] = base + mbcnt(m);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kBins) gbase[threadIdx.x] = atomicAdd(&ctl->bin_count[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+        for (int k = 0; k < kBinPer; ++k)
+            if (bin[k] >= 0) bins[(size_t)bin[k] * bin_cap + gbase[bin[k]] + pos[k]] = slot[k];
+        __syncthreads();
     }
 }
 
@@ -215,29 +294,27 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
 // an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_shadow(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* shq,
-                                                      uint32_t* spill, size_t spill_stride,
+                                                      uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats) {
     __shared__ uint32_t stk[kStackK * 256];
     const uint32_t count = ctl->sh_count;
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
     TraceCount tc{0, 0, 0};
-    for (;;) {
-        const uint32_t base = wave_fetch(&ctl->fetch[1], 64);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        if (i < count) {
+    persistent_trace<true, COUNT>(
+        S, &ctl->fetch[1], count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<kStackK>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
-            const Ray r = {xyz(o4), xyz(P.sh_d[slot])};
-            float t = o4.w;
-            int hi, hp;
-            if (!trace<true, COUNT>(S, r, t, hi, hp, st, tc)) {
+            tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, S.exact_only != 0);
+            // a NaN t max fails every slab test of the reference: empty traversal
+            if (!(o4.w == o4.w)) tr.st.sp = 0;
+        },
+        [&](Traversal<kStackK>& tr, uint32_t i, bool fin, bool found) {
+            if (fin && !found) {
+                const uint32_t slot = shq[i];
                 const V3 a = xyz(P.nee_a[slot]), b = xyz(P.nee_b[slot]);
                 P.L[slot] = f4(fma3(a, b, xyz(P.L[slot])), 0.0f);
             }
-        }
-    }
+        });
     if (COUNT) {
         wave_add_stats(&stats[3], tc.nodes);
         wave_add_stats(&stats[4], tc.tris);
@@ -285,27 +362,27 @@ __global__ void k_shade_normal(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* 
 // First hit of one sample per pixel: (instance, primitive, t) in pixel order.
 __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int height, int sample_id, int spp,
                                                    int frame_id, int* out_inst, int* out_prim, float* out_t,
-                                                   uint32_t* spill, size_t spill_stride) {
+                                                   uint32_t* spill, size_t spill_stride, uint32_t* fetch) {
     __shared__ uint32_t stk[kStackK * 256];
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack<kStackK> st{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
-    const size_t n = (size_t)width * height;
+    const uint32_t n = (uint32_t)width * (uint32_t)height;
     TraceCount tc{0, 0, 0};
-    for (size_t p = gid; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(p % width), y = (int)(p / width);
-        uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
-        V2 uPix, uLens;
-        uPix.x = lcg(rng); uPix.y = lcg(rng);
-        uLens.x = lcg(rng); uLens.y = lcg(rng);
-        const Ray r = camera_ray(S, x, y, uPix, uLens);
-        float t = kInf;
-        int hi = -1, hp = -1;
-        if (trace<false, false>(S, r, t, hi, hp, st, tc)) {
-            out_inst[p] = hi; out_prim[p] = hp; out_t[p] = t;
-        } else {
-            out_inst[p] = -1; out_prim[p] = -1; out_t[p] = kInf;
-        }
-    }
+    persistent_trace<false, false>(
+        S, fetch, n, stk, spill, spill_stride, 16u, tc,
+        [&](Traversal<kStackK>& tr, uint32_t p) {
+            const int x = (int)(p % (uint32_t)width), y = (int)(p / (uint32_t)width);
+            uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
+            V2 uPix, uLens;
+            uPix.x = lcg(rng); uPix.y = lcg(rng);
+            uLens.x = lcg(rng); uLens.y = lcg(rng);
+            tr.start(camera_ray(S, x, y, uPix, uLens), kInf, S.exact_only != 0);
+        },
+        [&](Traversal<kStackK>& tr, uint32_t p, bool fin, bool) {
+            if (fin) {
+                out_inst[p] = tr.hit_inst;
+                out_prim[p] = tr.hit_prim;
+                out_t[p] = tr.hit_inst >= 0 ? tr.t : kInf;
+            }
+        });
 }
 
 // ------------------------------------------------------------------ launch
@@ -317,14 +394,18 @@ void launch_reset(Ctl* ctl, int nxt, hipStream_t st) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt);
 }
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride,
+                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride, uint32_t refill,
                           unsigned long long* stats, int grid, hipStream_t st) {
     if (count)
         hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
-                           spill, spill_stride, stats);
+                           spill, spill_stride, refill, stats);
     else
         hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
-                           spill, spill_stride, stats);
+                           spill, spill_stride, refill, stats);
+}
+void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
+                int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap);
 }
 void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
                   uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec, int depth, int max_depth,
@@ -333,13 +414,14 @@ void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t*
                        depth, max_depth, rr_depth);
 }
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         uint32_t* spill, size_t spill_stride, unsigned long long* stats, int grid, hipStream_t st) {
+                         uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
+                         hipStream_t st) {
     if (count)
         hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
-                           stats);
+                           refill, stats);
     else
         hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
-                           stats);
+                           refill, stats);
 }
 void launch_film(const PathBuf& P, float* film, const ChunkInfo& c, float inv_spp, hipStream_t st) {
     hipLaunchKernelGGL(k_film, dim3((c.width + 255) / 256, c.height), dim3(256), 0, st, P, film, c, inv_spp);
@@ -349,9 +431,9 @@ void launch_shade_normal(const DevScene& S, const PathBuf& P, Ctl* ctl, const ui
     hipLaunchKernelGGL(k_shade_normal, dim3(grid), dim3(256), 0, st, S, P, ctl, bins, bin_cap);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
-                      float* t, uint32_t* spill, size_t spill_stride, int grid, hipStream_t st) {
+                      float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_first_hit, dim3(grid), dim3(256), 0, st, S, w, h, sample_id, spp, frame_id, inst, prim, t,
-                       spill, spill_stride);
+                       spill, spill_stride, fetch);
 }
 
 }  // namespace bnd

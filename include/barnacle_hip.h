@@ -272,7 +272,8 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
  *   "timing"               1: record a HIP event pair around every kernel
  *                          launch of the render calls, on the context's stream
  *   "chunk_slots"          path slots in flight per bounce loop (default 8M)
- *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels */
+ *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
+ *   "refill"               idle lanes of a wave that trigger a refill (1..64) */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {
