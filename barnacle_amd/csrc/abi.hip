@@ -19,6 +19,7 @@
 #include "abi_internal.hpp"
 #include "host_scene.hpp"
 #include "kernels.hpp"
+#include "kernels_mlt.hpp"
 
 using bnh::Error;
 
@@ -390,6 +391,130 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
     }
 }
 
+// PSSMLTIntegrator.Render (Extensions/Integrator/PSSMLT.fs:379-414): the
+// bootstrap paths and the chains run as wavefronts on the device; the mean
+// bootstrap luminance B (Array.average, sequential) and the bootstrap alias
+// table are formed on the host between the two phases, as the reference does.
+void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_host, bn_mlt_stats* stats) {
+    if (p.width <= 0 || p.height <= 0) throw Error(BN_E_INVALID, "film size must be positive");
+    if (p.n_bootstrap <= 0 || p.n_chains <= 0 || p.spp < 0 || p.max_depth < 0)
+        throw Error(BN_E_INVALID, "n-bootstrap and n-chains must be positive, spp and max-depth >= 0");
+    if (p.strategy != BN_MLT_GAUSSIAN && p.strategy != BN_MLT_KELEMEN) throw Error(BN_E_INVALID, "unknown strategy");
+    if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
+    hipStream_t s = c->stream;
+    camera_constants(sc, p.width, p.height);
+    const int nx = 4 + 7 * p.max_depth;
+    const size_t cap = std::min<size_t>(std::max(p.n_bootstrap, p.n_chains), c->mlt_chunk);
+    ensure_paths(c, cap);
+    ensure_spill(c);
+    if (cap > c->mlt_cap || nx > c->mlt_nx) {
+        const size_t k = std::max(cap, c->mlt_cap);
+        c->m_inner.ensure(k * 4); c->m_outer.ensure(k * 4); c->m_state.ensure(k * 16); c->m_bid.ensure(k * 4);
+        c->m_pix_new.ensure(k * 4); c->m_pix_cur.ensure(k * 4); c->m_Lcur.ensure(k * 16); c->m_rad.ensure(k * 16);
+        c->m_acc.ensure(k * 4);
+        c->m_xs.ensure(k * (size_t)std::max(nx, c->mlt_nx) * 16);
+        c->mlt_cap = k;
+        c->mlt_nx = std::max(nx, c->mlt_nx);
+    }
+    bnd::MltBuf M;
+    M.inner = (uint32_t*)c->m_inner.p; M.outer = (uint32_t*)c->m_outer.p; M.state = (int4*)c->m_state.p;
+    M.bid = (int*)c->m_bid.p; M.pix_new = (int*)c->m_pix_new.p; M.pix_cur = (int*)c->m_pix_cur.p;
+    M.Lcur = (float4*)c->m_Lcur.p; M.rad = (float4*)c->m_rad.p; M.accepted = (int*)c->m_acc.p;
+    M.xs = (float4*)c->m_xs.p; M.cap = (uint32_t)cap;
+    M.strategy = p.strategy; M.sigma = p.sigma; M.eps_min = p.eps_min; M.eps_max = p.eps_max;
+    const size_t film_n = (size_t)p.width * p.height * 3;
+    c->film.ensure(film_n * 4);
+    c->ctl.ensure(sizeof(bnd::Ctl));
+    c->m_weights.ensure((size_t)p.n_bootstrap * 4);
+    c->m_accepted_out.ensure((size_t)p.n_chains * 4);
+    hip_check(hipMemsetAsync(c->film.p, 0, film_n * 4, s), "memset film");     // Film.Clear
+    bnd::PathBuf P = path_buf(c);
+    bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
+    const int gt = grid_trace(c), gs = c->n_cu * 8;
+    KernelTimer& T = c->timer;
+    bnd::MltParamsDev dp{p.width, p.height, p.frame_id, p.n_bootstrap, p.large_step_prob, 0.0f, 0.0f};
+    const uint32_t bin_cap = (uint32_t)cap;
+
+    // One MLT path per slot [0, n): start, bounce loop.
+    auto run_paths = [&](bool fresh, int id0, int n) {
+        hip_check(hipMemsetAsync(ctl, 0, sizeof(bnd::Ctl), s), "memset ctl");
+        { Timed tm(T, "mlt_start", s);
+          bnd::launch_mlt_start(fresh, sc.dev, P, M, dp, ctl, (uint32_t*)c->queue[0].p, id0, n, gs, s); }
+        int cur = 0;
+        for (int depth = 0; depth < p.max_depth; ++depth) {
+            const int nxt = 1 - cur;
+            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
+            { Timed tm(T, "trace_closest", s);
+              bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
+                                        (uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->spill.p, c->spill_stride,
+                                        (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
+            { Timed tm(T, "bin", s);
+              bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
+            { Timed tm(T, "shade_mlt", s);
+              bnd::launch_shade_mlt(fresh, sc.dev, P, M, ctl, (const uint32_t*)c->bins.p, bin_cap,
+                                    (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p, (const int4*)sc.lrec.p,
+                                    depth, p.max_depth, p.rr_depth, gs, s); }
+            { Timed tm(T, "trace_shadow", s);
+              bnd::launch_trace_shadow(false, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
+                                       c->spill_stride, (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt,
+                                       s); }
+            cur = nxt;
+        }
+    };
+    c->stats.ensure(8 * sizeof(unsigned long long));
+
+    // PSSMLT.fs:382-392: bootstrap
+    for (int id0 = 0; id0 < p.n_bootstrap; id0 += (int)cap) {
+        const int n = std::min<int>((int)cap, p.n_bootstrap - id0);
+        run_paths(true, id0, n);
+        bnd::launch_mlt_boot_done(P, (float*)c->m_weights.p, id0, n, s);
+    }
+    c->last_bootstrap.resize(p.n_bootstrap);
+    hip_check(hipMemcpyAsync(c->last_bootstrap.data(), c->m_weights.p, (size_t)p.n_bootstrap * 4,
+                             hipMemcpyDeviceToHost, s), "weights D2H");
+    hip_check(hipStreamSynchronize(s), "bootstrap");
+    float sum = 0.0f;
+    for (float w : c->last_bootstrap) sum = sum + w;              // Array.average (PSSMLT.fs:394)
+    const float B = sum / (float)p.n_bootstrap;
+    const uint64_t mpc64 = ((uint64_t)p.spp * (uint64_t)p.width * (uint64_t)p.height + (uint64_t)p.n_chains - 1) /
+                           (uint64_t)p.n_chains;
+    if (mpc64 > 0x7fffffffull) throw Error(BN_E_CAPACITY, "mutations per chain exceed int32");
+    const int mpc = (int)mpc64;
+    c->last_chain_accepts.assign(p.n_chains, 0);
+    int64_t accepted = 0;
+    if (B != 0.0f) {                                              // PSSMLT.fs:396-397
+        const bnh::AliasTable tab = bnh::build_alias(c->last_bootstrap.data(), p.n_bootstrap);   // :399
+        c->m_aprob.upload(tab.prob.data(), tab.prob.size() * 4, s);
+        c->m_aidx.upload(tab.alias.data(), tab.alias.size() * 4, s);
+        dp.inv_b = 1.0f / B;
+        dp.inv_eff = 1.0f / ((float)mpc * (float)p.n_chains / (float)(p.width * p.height));
+        for (int c0 = 0; c0 < p.n_chains; c0 += (int)cap) {
+            const int n = std::min<int>((int)cap, p.n_chains - c0);
+            bnd::launch_mlt_chain_init(M, dp, (const float*)c->m_aprob.p, (const int*)c->m_aidx.p, c0, n, s);
+            run_paths(false, c0, n);                                 // replay of the bootstrap path
+            bnd::launch_mlt_replay_done(P, M, dp, c0, n, s);
+            for (int m = 0; m < mpc; ++m) {
+                run_paths(false, c0, n);
+                { Timed tm(T, "mlt_accept", s); bnd::launch_mlt_accept(P, M, dp, (float*)c->film.p, n, s); }
+            }
+            bnd::launch_mlt_finish(M, dp, (float*)c->film.p, (int*)c->m_accepted_out.p, c0, n, s);
+        }
+        hip_check(hipMemcpyAsync(c->last_chain_accepts.data(), c->m_accepted_out.p, (size_t)p.n_chains * 4,
+                                 hipMemcpyDeviceToHost, s), "accepts D2H");
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+    if (film_host) hip_check(hipMemcpyAsync(film_host, c->film.p, film_n * 4, hipMemcpyDeviceToHost, s), "film D2H");
+    hip_check(hipStreamSynchronize(s), "pssmlt");
+    c->timer.resolve();
+    for (int32_t a : c->last_chain_accepts) accepted += a;
+    if (stats) {
+        stats->accepted = accepted;
+        stats->proposed = B != 0.0f ? (int64_t)mpc * p.n_chains : 0;
+        stats->b = B;
+        stats->mutations_per_chain = mpc;
+    }
+}
+
 }  // namespace bnx
 
 using namespace bnx;
@@ -654,30 +779,28 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* p, int32_t samp
 
 }  // extern "C"
 
-extern "C" int bn_render_pssmlt(bn_ctx* ctx, int32_t scene, const bn_mlt_params* params, float* film,
-                                bn_mlt_stats* stats) {
-    return guard([&] {
-        (void)ctx; (void)scene; (void)params; (void)film; (void)stats;
-        throw Error(BN_E_UNSUPPORTED, "bn_render_pssmlt: not built yet");
-    });
-}
-
 extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
     return guard([&] {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
         if (!c || !key) throw Error(BN_E_INVALID, "bad arguments");
         const std::string k(key);
-        if (k == "timing") c->timer.on = value != 0;
-        else if (k == "chunk_slots") {
+        if (k == "timing") {
+            c->timer.on = value != 0;
+        } else if (k == "chunk_slots") {
             if (value < 64 || value > ((int64_t)1 << 30)) throw Error(BN_E_INVALID, "chunk_slots out of range");
             c->chunk_slots = (size_t)value;
         } else if (k == "trace_blocks_per_cu") {
             if (value < 1 || value > 5) throw Error(BN_E_INVALID, "trace_blocks_per_cu must be in [1, 5]");
             c->trace_blocks_per_cu = (int)value;
+        } else if (k == "mlt_chunk") {
+            if (value < 64 || value > ((int64_t)1 << 26)) throw Error(BN_E_INVALID, "mlt_chunk out of range");
+            c->mlt_chunk = (size_t)value;
         } else if (k == "refill") {
             if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
             c->refill = (int)value;
-        } else throw Error(BN_E_INVALID, "unknown option " + k);
+        } else {
+            throw Error(BN_E_INVALID, "unknown option " + k);
+        }
     });
 }
 
@@ -702,5 +825,32 @@ extern "C" int bn_reset_kernel_times(bn_ctx* ctx) {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
         if (!c) throw Error(BN_E_INVALID, "null context");
         c->timer.totals.clear();
+    });
+}
+
+extern "C" int bn_render_pssmlt(bn_ctx* ctx, int32_t scene, const bn_mlt_params* params, float* film,
+                                bn_mlt_stats* stats) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!params) throw Error(BN_E_INVALID, "null params");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        render_pssmlt(c, s, *params, film, stats);
+    });
+}
+
+extern "C" int bn_pssmlt_last(bn_ctx* ctx, float* bootstrap, int32_t n_bootstrap, int32_t* chain_accepts,
+                              int32_t n_chains) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c) throw Error(BN_E_INVALID, "null context");
+        if (bootstrap) {
+            if (n_bootstrap != (int32_t)c->last_bootstrap.size()) throw Error(BN_E_INVALID, "bootstrap size mismatch");
+            std::memcpy(bootstrap, c->last_bootstrap.data(), c->last_bootstrap.size() * 4);
+        }
+        if (chain_accepts) {
+            if (n_chains != (int32_t)c->last_chain_accepts.size()) throw Error(BN_E_INVALID, "chain count mismatch");
+            std::memcpy(chain_accepts, c->last_chain_accepts.data(), c->last_chain_accepts.size() * 4);
+        }
     });
 }

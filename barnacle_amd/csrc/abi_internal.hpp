@@ -10,6 +10,7 @@
 #include "../../include/barnacle_hip.h"
 #include "host_scene.hpp"
 #include "kernels.hpp"
+#include "kernels_mlt.hpp"
 
 namespace bnx {
 
@@ -75,6 +76,14 @@ struct Ctx {
     size_t spill_stride = 0;
     DevBuf out_a, out_b, out_c;
     KernelTimer timer;
+    // PSSMLT chain state
+    size_t mlt_cap = 0;
+    int mlt_nx = 0;
+    size_t mlt_chunk = (size_t)1 << 20;   // chains (or bootstrap paths) in flight
+    DevBuf m_inner, m_outer, m_state, m_bid, m_pix_new, m_pix_cur, m_Lcur, m_rad, m_acc, m_xs;
+    DevBuf m_weights, m_aprob, m_aidx, m_accepted_out;
+    std::vector<float> last_bootstrap;     // of the last bn_render_pssmlt
+    std::vector<int32_t> last_chain_accepts;
     ~Ctx() {
         if (stream) (void)hipStreamDestroy(stream);
     }

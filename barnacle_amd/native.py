@@ -55,6 +55,7 @@ _SIGS = {
     "bn_reset_kernel_times": ([C.c_void_p], C.c_int),
     "bn_render_pssmlt": ([C.c_void_p, C.c_int32, C.POINTER(abi.MLTParams), f32p, C.POINTER(abi.MLTStats)],
                          C.c_int),
+    "bn_pssmlt_last": ([C.c_void_p, f32p, C.c_int32, i32p, C.c_int32], C.c_int),
 }
 
 
@@ -171,6 +172,14 @@ class DeviceScene:
         check(self.ctx.lib.bn_render_pssmlt(self.ctx.h, self.id, C.byref(params), _p(film, f32p), C.byref(st)))
         return film, {"accepted": st.accepted, "proposed": st.proposed, "b": np.float32(st.b),
                       "mutations_per_chain": st.mutations_per_chain}
+
+    def pssmlt_last(self, params: abi.MLTParams):
+        """(bootstrap luminances, per-chain accept counts) of the last PSSMLT render."""
+        boot = np.zeros(params.n_bootstrap, np.float32)
+        acc = np.zeros(params.n_chains, np.int32)
+        check(self.ctx.lib.bn_pssmlt_last(self.ctx.h, _p(boot, f32p), params.n_bootstrap, _p(acc, i32p),
+                                          params.n_chains))
+        return boot, acc
 
     def export(self):
         cap = 4 * max(1, len(self.sc.instances)) * 64 + 16

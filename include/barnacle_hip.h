@@ -273,7 +273,8 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
  *                          launch of the render calls, on the context's stream
  *   "chunk_slots"          path slots in flight per bounce loop (default 8M)
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
- *   "refill"               idle lanes of a wave that trigger a refill (1..64) */
+ *   "refill"               idle lanes of a wave that trigger a refill (1..64)
+ *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {
@@ -291,6 +292,13 @@ int bn_reset_kernel_times(bn_ctx* ctx);
  * atomics (the reference's Film.Accumulate is an unsynchronised add). */
 int bn_render_pssmlt(bn_ctx* ctx, int32_t scene, const bn_mlt_params* params,
                      float* film_rgb, bn_mlt_stats* stats);
+
+/* Per-path detail of the last bn_render_pssmlt: the bootstrap luminances
+ * (BootstrapWeights, PSSMLT.fs:163) and the accepted-mutation count of every
+ * chain (the per-chain terms of AcceptedMutationCount, :376).  Either output
+ * may be NULL; sizes must equal n_bootstrap / n_chains of that call. */
+int bn_pssmlt_last(bn_ctx* ctx, float* bootstrap, int32_t n_bootstrap,
+                   int32_t* chain_accepts, int32_t n_chains);
 
 #ifdef __cplusplus
 }

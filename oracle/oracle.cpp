@@ -1132,6 +1132,7 @@ int orc_struct_sizes(int32_t* out, int32_t n) {
 }
 
 uint32_t orc_xxhash32_two(uint32_t x, uint32_t y) { return xx2(x, y); }
+float orc_erfinv(float x) { return erfInv(x); }
 uint32_t orc_xxhash32_three(uint32_t x, uint32_t y, uint32_t z) { return xx3(x, y, z); }
 float orc_lcg(uint32_t* state) { Sampler s{*state}; float f = s.next(); *state = s.s; return f; }
 

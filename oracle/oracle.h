@@ -73,6 +73,8 @@ int  orc_struct_sizes(int32_t* out, int32_t n);
 uint32_t orc_xxhash32_two(uint32_t x, uint32_t y);
 uint32_t orc_xxhash32_three(uint32_t x, uint32_t y, uint32_t z);
 float    orc_lcg(uint32_t* state);
+/* ErfInv of the Gaussian mutation (PSSMLT.fs:69-94). */
+float    orc_erfinv(float x);
 
 /* AliasTable.fs:14-62 */
 int orc_alias_build(const float* w, int32_t n, int32_t* alias, float* prob, float* pdf);

@@ -54,6 +54,8 @@ class Oracle:
         L.orc_xxhash32_three.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_lcg.restype = C.c_float
         L.orc_lcg.argtypes = [u32p]
+        L.orc_erfinv.restype = C.c_float
+        L.orc_erfinv.argtypes = [C.c_float]
         L.orc_alias_sample.argtypes = [i32p, f32p, f32p, C.c_int32, C.c_float, i32p, f32p]
         L.orc_scene_create.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(C.c_void_p)]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
@@ -85,6 +87,9 @@ class Oracle:
         s = C.c_uint32(state)
         out = [self.lib.orc_lcg(C.byref(s)) for _ in range(n)]
         return np.array(out, np.float32), s.value
+
+    def erfinv(self, x):
+        return np.float32(self.lib.orc_erfinv(float(x)))
 
     def alias_build(self, w):
         w = np.ascontiguousarray(w, np.float32)
