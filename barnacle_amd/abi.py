@@ -95,9 +95,17 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", i64), ("total_ms", C.c_double)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", i32), ("params", PTParams)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("rounds", i32), ("graph", i32)]
+
+
 # Declaration order of include/barnacle_hip.h, as bn_struct_sizes() reports it.
 STRUCT_ORDER = [Primitive, Material, Light, Instance, Transform, Node, Camera,
                 SceneDesc, PTParams, MLTParams, MLTStats, TraceStats, BVHNode,
-                InstanceUpload]
+                InstanceUpload, BatchItem, BatchStats, KernelTime]
 
 BN_OK, BN_E_INVALID, BN_E_NO_LIGHT, BN_E_DEVICE, BN_E_UNSUPPORTED, BN_E_CAPACITY = 0, -1, -2, -3, -4, -5

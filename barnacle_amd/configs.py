@@ -41,32 +41,32 @@ def icosphere(freq: int):
     t = (1.0 + 5 ** 0.5) / 2.0
     base = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
                      [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
-    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
-             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
-             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
-    key_to_id, verts, tris = {}, [], []
-
-    def vid(p):
-        p = p / np.linalg.norm(p)
-        k = tuple(np.round(p * 1e9).astype(np.int64))
-        if k not in key_to_id:
-            key_to_id[k] = len(verts)
-            verts.append(p)
-        return key_to_id[k]
-
-    for a, b, c in faces:
-        A, B, Cc = base[a], base[b], base[c]
-        grid = {}
-        for i in range(freq + 1):
-            for j in range(freq + 1 - i):
-                grid[i, j] = vid(A + (B - A) * (i / freq) + (Cc - A) * (j / freq))
-        for i in range(freq):
-            for j in range(freq - i):
-                tris.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
-                if i + j < freq - 1:
-                    tris.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
-    V = np.array(verts)
-    T = np.array(tris, np.int64)
+    faces = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+                      (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+                      (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)])
+    ii, jj = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    keep = ii + jj <= freq
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((freq + 1, freq + 1), np.int64)
+    lid[ii, jj] = np.arange(len(ii))
+    A, B, Cc = base[faces[:, 0]], base[faces[:, 1]], base[faces[:, 2]]
+    P = (A[:, None, :] + (B - A)[:, None, :] * (ii / freq)[None, :, None] +
+         (Cc - A)[:, None, :] * (jj / freq)[None, :, None]).reshape(-1, 3)
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    keys = np.round(P * 1e9).astype(np.int64)
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)                      # vertices numbered by first appearance
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    V = P[first[order]]
+    gid = rank[inv.reshape(-1)].reshape(20, -1)
+    i, j = np.meshgrid(np.arange(freq), np.arange(freq), indexing="ij")
+    up = (i + j) <= freq - 1
+    dn = (i + j) < freq - 1
+    t_up = np.stack([lid[i[up], j[up]], lid[i[up] + 1, j[up]], lid[i[up], j[up] + 1]], 1)
+    t_dn = np.stack([lid[i[dn] + 1, j[dn]], lid[i[dn] + 1, j[dn] + 1], lid[i[dn], j[dn] + 1]], 1)
+    local = np.concatenate([t_up, t_dn])
+    T = np.concatenate([gid[f][local] for f in range(20)])
     n = np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]])
     flip = np.einsum("ij,ij->i", n, V[T].mean(axis=1)) < 0
     T[flip] = T[flip][:, [0, 2, 1]]
@@ -193,3 +193,79 @@ def caustic_scene(width=1024, height=1024, spp=256, max_depth=16, freq=32,
                           "mutation-strategy": "Gaussian", "mutation-sigma": sigma,
                           "large-step-prob": large_step}
     return info
+
+
+def batch_scene(seed: int, max_res: int = 1024, max_spp: int = 64, tri_range=(1000, 500000),
+                max_instances: int = 256):
+    """One scene of the 32-scene batch: a ground quad, a pool of at most 8
+    displaced icospheres (log-uniform triangle counts) instanced 1-256 times
+    with random affine transforms (scale, rotation in radians, translation),
+    a random mix of lambertian / mirror / dielectric (IOR 1.5) / pbr materials
+    and 1-8 quad area lights.  Resolution from {256, 512, 768, 1024}^2 (capped
+    at max_res), spp 4-64 (capped at max_spp), depth 4-16."""
+    rng = np.random.default_rng(2024 + seed)
+    res = int(rng.choice([256, 512, 768, 1024]))
+    res = min(res, max_res)
+    spp = int(min(rng.integers(4, 65), max_spp))
+    depth = int(rng.integers(4, 17))
+    n_blas = int(rng.integers(1, 9))
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    prims, mats = [], []
+    for b in range(n_blas):
+        tris = float(np.exp(rng.uniform(lo, hi)))
+        freq = max(2, int(round(np.sqrt(tris / 20.0))))
+        prims.append(displaced_sphere(freq, 0.08, int(rng.integers(0, 1 << 30))))
+    kinds = ["lambertian", "mirror", "dielectric", "pbr"]
+    for m in range(8):
+        k = kinds[int(rng.integers(0, 4))]
+        alb = [float(np.float32(x)) for x in rng.uniform(0.2, 0.9, 3)]
+        if k == "pbr":
+            mats.append({"type": k, "albedo": alb, "metallic": float(rng.integers(0, 2)),
+                         "roughness": float(np.float32(rng.uniform(0.05, 0.8)))})
+        elif k == "dielectric":
+            mats.append({"type": k, "albedo": alb, "ior": 1.5})
+        else:
+            mats.append({"type": k, "albedo": alb})
+    mats.append({"type": "lambertian", "albedo": WHITE})              # ground
+    n_inst = int(np.clip(np.round(np.exp(rng.uniform(0, np.log(max_instances)))), 1, max_instances))
+    R = 100.0
+    insts, transforms, node_children, nodes = [], [], [], [{"instances": [], "children": []}]
+    for i in range(n_inst):
+        s_ = float(np.float32(min(0.6 * R, rng.uniform(0.2, 0.5) * 2 * R / n_inst ** (1 / 3))))
+        sc = [s_ * float(np.float32(x)) for x in rng.uniform(0.7, 1.3, 3)]
+        rot = [float(np.float32(x)) for x in rng.uniform(-np.pi, np.pi, 3)]
+        tr = [float(np.float32(x)) for x in (rng.uniform(-R, R), s_ + rng.uniform(0.0, 2 * R), rng.uniform(-R, R))]
+        transforms.append({"keyframes": [{"time": 0.0, "scale": sc, "rotation": rot, "translation": tr}]})
+        insts.append({"primitive": int(rng.integers(0, n_blas)), "material": int(rng.integers(0, 8))})
+        nodes.append({"instances": [len(insts) - 1], "transform": len(transforms) - 1})
+        node_children.append(len(nodes) - 1)
+    G = 4 * R
+    prims.append(_quad((-G, 0, -G), (G, 0, -G), (G, 0, G), (-G, 0, G)))
+    insts.append({"primitive": len(prims) - 1, "material": 8})
+    nodes[0]["instances"].append(len(insts) - 1)
+    lights = []
+    n_lights = int(rng.integers(1, 9))
+    for l in range(n_lights):
+        cx, cz = rng.uniform(-R, R, 2)
+        h = float(rng.uniform(2.2 * R, 3.0 * R))
+        w = float(rng.uniform(10, 40))
+        prims.append(_quad((cx - w, h, cz - w), (cx + w, h, cz - w), (cx + w, h, cz + w), (cx - w, h, cz + w)))
+        e = [float(np.float32(x)) for x in rng.uniform(5, 20, 3)]
+        lights.append({"type": "diffuse", "emission": e})
+        insts.append({"primitive": len(prims) - 1, "light": l})
+        nodes[0]["instances"].append(len(insts) - 1)
+    transforms.append({"keyframes": [{"time": 0.0, "rotation": [-0.35, 0.0, 0.0],
+                                      "translation": [0.0, 1.6 * R, 4.2 * R]}]})
+    nodes.append({"transform": len(transforms) - 1, "has-camera": True})
+    node_children.append(len(nodes) - 1)
+    nodes[0]["children"] = node_children
+    return {"root": 0, "nodes": nodes, "instances": insts, "transforms": transforms, "primitives": prims,
+            "materials": mats, "lights": lights,
+            "integrator": {"type": "path-tracing", "spp": spp, "max-depth": depth, "rr-depth": 5},
+            "camera": {"type": "pinhole", "fov": 50.0, "aspect-ratio": 1.0},
+            "film": {"width": res, "height": res, "tone-mapping": "aces"}}
+
+
+def config2(n: int = 32, **kw):
+    """The 32-scene batch (seeds 2024..2055)."""
+    return [batch_scene(i, **kw) for i in range(n)]

@@ -234,7 +234,7 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
     sc.dev.insts = (const bnd::DevInst*)sc.insts.p;
     sc.dev.mats = (const float4*)sc.mats.p;
     sc.dev.lightdefs = (const float4*)sc.lightdefs.p;
-    sc.dev.light_insts = nullptr;
+    sc.dev.lrec = (const int4*)sc.lrec.p;
     sc.dev.n_lights = sc.n_lights;
     sc.dev.alias_prob = (const float*)sc.aprob.p;
     sc.dev.alias_idx = (const int*)sc.aidx.p;
@@ -281,12 +281,14 @@ void ensure_paths(Ctx* c, size_t cap) {
     auto f4 = [&](DevBuf& b) { b.ensure(cap * sizeof(float4)); };
     f4(c->p_o); f4(c->p_d); f4(c->p_beta); f4(c->p_L); f4(c->p_sh_o); f4(c->p_sh_d); f4(c->p_nee_a); f4(c->p_nee_b);
     c->p_rng.ensure(cap * 4);
+    c->p_item.ensure(cap * 4);
     c->p_hit.ensure(cap * sizeof(int4));
     c->queue[0].ensure(cap * 4);
     c->queue[1].ensure(cap * 4);
     c->shq.ensure(cap * 4);
     c->bins.ensure(cap * 4 * bnd::kBins);
     c->path_cap = cap;
+    c->graphs.clear();                   // captured graphs point at the old buffers
 }
 
 bnd::PathBuf path_buf(Ctx* c) {
@@ -295,30 +297,80 @@ bnd::PathBuf path_buf(Ctx* c) {
     P.rng = (uint32_t*)c->p_rng.p; P.hit = (int4*)c->p_hit.p;
     P.sh_o = (float4*)c->p_sh_o.p; P.sh_d = (float4*)c->p_sh_d.p;
     P.nee_a = (float4*)c->p_nee_a.p; P.nee_b = (float4*)c->p_nee_b.p;
+    P.item = (int*)c->p_item.p;
     return P;
 }
 
 void ensure_spill(Ctx* c) {
     const size_t lanes = (size_t)grid_trace(c) * bnd::kBlock;
-    c->spill.ensure(lanes * (bnd::kStackMax - bnd::kStackK) * 4);
+    const size_t need = lanes * (bnd::kStackMax - bnd::kStackK) * 4;
+    if (need > c->spill.bytes) c->graphs.clear();
+    c->spill.ensure(need);
     c->spill_stride = lanes;
 }
 
-// The bounce loop of one frame of PathTracingIntegrator (or the Normal view).
-void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, float* film_host, bn_trace_stats* stats) {
+bnd::Buffers buffers(Ctx* c) {
+    bnd::Buffers B;
+    B.queue[0] = (uint32_t*)c->queue[0].p;
+    B.queue[1] = (uint32_t*)c->queue[1].p;
+    B.shq = (uint32_t*)c->shq.p;
+    B.bins = (uint32_t*)c->bins.p;
+    B.bin_cap = (uint32_t)c->path_cap;
+    B.spill = (uint32_t*)c->spill.p;
+    B.spill_stride = c->spill_stride;
+    B.stats = (unsigned long long*)c->stats.p;
+    B.film = (float*)c->film.p;
+    return B;
+}
+
+// Timing (per launch) and exact per-bounce counters of a bounce loop.
+struct Hook : bnd::LaunchHook {
+    Ctx* c;
+    bnd::Ctl* ctl;
+    bn_trace_stats* acc;    // non-null: read the counters back after every bounce
+    Hook(Ctx* cx, bnd::Ctl* ct, bn_trace_stats* a) : c(cx), ctl(ct), acc(a) {}
+    void begin(const char* name, hipStream_t st) override { c->timer.begin(name, st); }
+    void end(hipStream_t st) override { c->timer.end(st); }
+    void after_bounce(int cur) override {
+        if (!acc) return;
+        bnd::Ctl h;
+        hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream), "ctl readback");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        acc->rays_closest += h.q_count[cur];
+        acc->rays_shadow += h.sh_count;
+        for (int b = 0; b < bnd::kBins; ++b) acc->vertices += h.bin_count[b];
+    }
+};
+
+bnd::RoundItem plain_item(const bn_pt_params& p) {
+    bnd::RoundItem it{};
+    it.width = p.width; it.height = p.height;
+    it.tiles_x = (p.width + 7) / 8;
+    it.p_pad = it.tiles_x * ((p.height + 7) / 8) * 64;
+    it.spp = p.spp; it.frame_id = p.frame_id;
+    it.max_depth = p.max_depth; it.rr_depth = p.rr_depth;
+    it.inv_spp = 1.0f / (float)p.spp;                     // MathF.ReciprocalEstimate(spp)
+    it.pix_end = p.width * p.height;
+    return it;
+}
+
+void check_pt_params(const bn_pt_params& p) {
     if (p.width <= 0 || p.height <= 0) throw Error(BN_E_INVALID, "film size must be positive");
     if (p.spp < 0 || p.max_depth < 0) throw Error(BN_E_INVALID, "spp and max-depth must be >= 0");
+    if ((int64_t)p.width * p.height > ((int64_t)1 << 28)) throw Error(BN_E_CAPACITY, "film larger than 2^28 pixels");
+}
+
+// One frame of PathTracingIntegrator (or the Normal view), in chunks of
+// whole samples of all pixels, chunk after chunk in sample order.
+void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, float* film_host, bn_trace_stats* stats) {
+    check_pt_params(p);
     if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
     hipStream_t s = c->stream;
     camera_constants(sc, p.width, p.height);
-    bnd::ChunkInfo ci{};
-    ci.width = p.width; ci.height = p.height;
-    ci.tiles_x = (p.width + 7) / 8; ci.tiles_y = (p.height + 7) / 8;
-    ci.p_pad = ci.tiles_x * ci.tiles_y * 64;
-    ci.spp = p.spp; ci.frame_id = p.frame_id;
-    const size_t want = std::max<size_t>((size_t)ci.p_pad, c->chunk_slots);
-    const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / ci.p_pad));
-    ensure_paths(c, (size_t)ns * ci.p_pad);
+    bnd::RoundItem it = plain_item(p);
+    const size_t want = std::max<size_t>((size_t)it.p_pad, c->chunk_slots);
+    const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / it.p_pad));
+    ensure_paths(c, (size_t)ns * it.p_pad);
     ensure_spill(c);
     const size_t film_n = (size_t)p.width * p.height * 3;
     c->film.ensure(film_n * 4);
@@ -326,68 +378,173 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
     c->stats.ensure(8 * sizeof(unsigned long long));
     hip_check(hipMemsetAsync(c->film.p, 0, film_n * 4, s), "memset film");
     hip_check(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), s), "memset stats");
-    const bool count = stats != nullptr;
     bn_trace_stats acc{};
     bnd::PathBuf P = path_buf(c);
     bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
-    const uint32_t bin_cap = (uint32_t)(ns * ci.p_pad);
-    const float inv_spp = 1.0f / (float)p.spp;
-    const int gt = grid_trace(c);
-    const int gs = c->n_cu * 8;
+    Hook hook(c, ctl, stats ? &acc : nullptr);
+    bnd::Launch L{};
+    L.scene = sc.dev;
+    L.grid_trace = grid_trace(c);
+    L.grid_shade = c->n_cu * 8;
+    L.refill = (uint32_t)c->refill;
+    L.stream = s;
+    L.n_pix = (uint32_t)(p.width * p.height);
+    const bnd::Buffers B = buffers(c);
     for (int s0 = 0; s0 < p.spp; s0 += ns) {
-        ci.s0 = s0;
-        ci.ns = std::min(ns, p.spp - s0);
-        hip_check(hipMemsetAsync(ctl, 0, sizeof(bnd::Ctl), s), "memset ctl");
-        const int nslots = ci.ns * ci.p_pad;
-        KernelTimer& T = c->timer;
-        { Timed tm(T, "raygen", s);
-          bnd::launch_raygen(sc.dev, P, ctl, (uint32_t*)c->queue[0].p, ci, std::min(gs, (nslots + 255) / 256), s); }
-        int cur = 0;
-        const int depth_n = normal ? std::min(1, p.max_depth) : p.max_depth;
-        for (int depth = 0; depth < depth_n; ++depth) {
-            const int nxt = 1 - cur;
-            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
-            { Timed tm(T, "trace_closest", s);
-              bnd::launch_trace_closest(count, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
-                                        (uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->spill.p, c->spill_stride,
-                                        (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
-            { Timed tm(T, "bin", s);
-              bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
-            if (normal) {
-                Timed tm(T, "shade_normal", s);
-                bnd::launch_shade_normal(sc.dev, P, ctl, (const uint32_t*)c->bins.p, bin_cap, gs, s);
-                break;
-            }
-            { Timed tm(T, "shade", s);
-              bnd::launch_shade(sc.dev, P, ctl, (const uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->queue[nxt].p,
-                                nxt, (uint32_t*)c->shq.p, (const int4*)sc.lrec.p, depth, p.max_depth, p.rr_depth,
-                                gs, s); }
-            { Timed tm(T, "trace_shadow", s);
-              bnd::launch_trace_shadow(count, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
-                                       c->spill_stride, (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
-            if (count) {
-                bnd::Ctl h;
-                hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "ctl readback");
-                hip_check(hipStreamSynchronize(s), "sync");
-                acc.rays_closest += h.q_count[cur];
-                acc.rays_shadow += h.sh_count;
-                for (int b = 0; b < bnd::kBins; ++b) acc.vertices += h.bin_count[b];
-            }
-            cur = nxt;
-        }
-        { Timed tm(T, "film", s); bnd::launch_film(P, (float*)c->film.p, ci, inv_spp, s); }
+        it.s0 = s0;
+        it.ns = std::min(ns, p.spp - s0);
+        it.slot_count = it.ns * it.p_pad;
+        L.item = it;
+        if (normal) bnd::enqueue_normal(L, P, ctl, B);
+        else bnd::enqueue_bounce_loop(false, L, P, ctl, B, p.max_depth, stats != nullptr, &hook);
     }
     hip_check(hipGetLastError(), "kernel launch");
     if (film_host) hip_check(hipMemcpyAsync(film_host, c->film.p, film_n * 4, hipMemcpyDeviceToHost, s), "film D2H");
     hip_check(hipStreamSynchronize(s), "render");
     c->timer.resolve();
-    if (count) {
+    if (stats) {
         unsigned long long h[8];
         hip_check(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost), "stats D2H");
         acc.nodes_closest = (int64_t)h[0]; acc.tris_closest = (int64_t)h[1]; acc.insts_closest = (int64_t)h[2];
         acc.nodes_shadow = (int64_t)h[3]; acc.tris_shadow = (int64_t)h[4]; acc.insts_shadow = (int64_t)h[5];
         acc.paths = (int64_t)p.width * p.height * p.spp;
         *stats = acc;
+    }
+}
+
+// Several renders in one launch sequence per round (see kernels_pt.hip).
+// Rounds pack whole samples of the items into the slot capacity, every item
+// in sample order; a round's bounce loop is captured once as a graph and
+// replayed with the next round table.
+void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films, bn_batch_stats* stats) {
+    if (!items || n <= 0) throw Error(BN_E_INVALID, "empty batch");
+    if (n > 4096) throw Error(BN_E_CAPACITY, "more than 4096 batch items");
+    hipStream_t s = c->stream;
+    std::vector<bnd::RoundItem> base(n);
+    std::vector<bnd::DevScene> scenes(n);
+    size_t film_total = 0;
+    int depth_n = 0;
+    size_t max_pad = 0;
+    for (int k = 0; k < n; ++k) {
+        const bn_pt_params& p = items[k].params;
+        check_pt_params(p);
+        SceneRec& sc = get_scene(c, items[k].scene);
+        if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
+        camera_constants(sc, p.width, p.height);
+        scenes[k] = sc.dev;                                   // camera constants of this item's film
+        base[k] = plain_item(p);
+        base[k].scene = k;
+        base[k].film_offset = (int)film_total;
+        film_total += (size_t)p.width * p.height * 3;
+        if (film_total > ((size_t)1 << 31)) throw Error(BN_E_CAPACITY, "batch films exceed 2^31 floats");
+        depth_n = std::max(depth_n, p.max_depth);
+        max_pad = std::max(max_pad, (size_t)base[k].p_pad);
+    }
+    const size_t cap = std::max(max_pad, c->chunk_slots);
+    ensure_paths(c, cap);
+    ensure_spill(c);
+    c->film.ensure(film_total * 4);
+    c->ctl.ensure(sizeof(bnd::Ctl));
+    c->stats.ensure(8 * sizeof(unsigned long long));
+    c->b_scenes.upload(scenes.data(), scenes.size() * sizeof(bnd::DevScene), s);
+    c->b_items.ensure((size_t)n * sizeof(bnd::RoundItem));
+    c->b_hdr.ensure(sizeof(bnd::RoundHeader));
+    hip_check(hipMemsetAsync(c->film.p, 0, film_total * 4, s), "memset film");
+    bnd::PathBuf P = path_buf(c);
+    bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
+    bnd::Launch L{};
+    L.hdr = (const bnd::RoundHeader*)c->b_hdr.p;
+    L.items = (const bnd::RoundItem*)c->b_items.p;
+    L.scenes = (const bnd::DevScene*)c->b_scenes.p;
+    L.grid_trace = grid_trace(c);
+    L.grid_shade = c->n_cu * 8;
+    L.refill = (uint32_t)c->refill;
+    L.stream = s;
+    const bnd::Buffers B = buffers(c);
+    hipGraphExec_t exec = nullptr;
+    if (c->use_graph) {
+        const std::string key = "batch:" + std::to_string(depth_n) + ":" + std::to_string(L.grid_trace) + ":" +
+                                std::to_string(L.refill) + ":" + std::to_string((uintptr_t)c->b_items.p) + ":" +
+                                std::to_string((uintptr_t)c->b_scenes.p) + ":" + std::to_string((uintptr_t)c->film.p);
+        auto f = c->graphs.find(key);
+        if (f == c->graphs.end()) {
+            hipGraph_t g;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            bnd::enqueue_bounce_loop(true, L, P, ctl, B, depth_n, false, nullptr);
+            hip_check(hipStreamEndCapture(s, &g), "end capture");
+            hipGraphExec_t e;
+            hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "graph instantiate");
+            hip_check(hipGraphDestroy(g), "graph destroy");
+            f = c->graphs.emplace(key, GraphExec(e)).first;
+        }
+        exec = f->second.e;
+    }
+    std::vector<int> next(n, 0);
+    std::vector<bnd::RoundItem> round;
+    int rounds = 0;
+    for (;;) {
+        round.clear();
+        size_t used = 0;
+        int active = 0;
+        for (int k = 0; k < n; ++k) active += next[k] < items[k].params.spp;
+        if (active == 0) break;
+        const size_t share = std::max<size_t>(1, cap / active);
+        // pass 1: a fair share for every unfinished item; pass 2: fill up
+        std::vector<int> take(n, 0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < n; ++k) {
+                const int left = items[k].params.spp - next[k] - take[k];
+                if (left <= 0) continue;
+                const size_t pad = (size_t)base[k].p_pad;
+                size_t fit = (cap - used) / pad;
+                if (pass == 0) fit = std::min(fit, std::max<size_t>(1, share / pad));
+                const int t = (int)std::min<size_t>((size_t)left, fit);
+                if (t <= 0) continue;
+                take[k] += t;
+                used += (size_t)t * pad;
+            }
+        int slot = 0, pix = 0;
+        for (int k = 0; k < n; ++k) {
+            if (take[k] == 0) continue;
+            bnd::RoundItem it = base[k];
+            it.s0 = next[k];
+            it.ns = take[k];
+            it.slot_base = slot;
+            it.slot_count = take[k] * it.p_pad;
+            pix += it.width * it.height;
+            it.pix_end = pix;
+            slot += it.slot_count;
+            next[k] += take[k];
+            round.push_back(it);
+        }
+        bnd::RoundHeader h{(int)round.size(), (uint32_t)slot, (uint32_t)pix, 0};
+        // the stream orders these copies before the round's kernels
+        hip_check(hipMemcpyAsync(c->b_items.p, round.data(), round.size() * sizeof(bnd::RoundItem),
+                                 hipMemcpyHostToDevice, s), "round table");
+        hip_check(hipMemcpyAsync(c->b_hdr.p, &h, sizeof(h), hipMemcpyHostToDevice, s), "round header");
+        if (exec) {
+            c->timer.begin("batch_round", s);
+            hip_check(hipGraphLaunch(exec, s), "graph launch");
+            c->timer.end(s);
+        } else {
+            Hook hook(c, ctl, nullptr);
+            bnd::enqueue_bounce_loop(true, L, P, ctl, B, depth_n, false, &hook);
+        }
+        // the host round table is reused next round: wait for the copies
+        hip_check(hipStreamSynchronize(s), "round");
+        ++rounds;
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+    for (int k = 0; k < n; ++k)
+        if (films && films[k])
+            hip_check(hipMemcpyAsync(films[k], (float*)c->film.p + base[k].film_offset,
+                                     (size_t)items[k].params.width * items[k].params.height * 12,
+                                     hipMemcpyDeviceToHost, s), "film D2H");
+    hip_check(hipStreamSynchronize(s), "batch");
+    c->timer.resolve();
+    if (stats) {
+        stats->rounds = rounds;
+        stats->graph = exec ? 1 : 0;
     }
 }
 
@@ -433,11 +590,11 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
     const int gt = grid_trace(c), gs = c->n_cu * 8;
     KernelTimer& T = c->timer;
     bnd::MltParamsDev dp{p.width, p.height, p.frame_id, p.n_bootstrap, p.large_step_prob, 0.0f, 0.0f};
-    const uint32_t bin_cap = (uint32_t)cap;
+    const uint32_t bin_cap = (uint32_t)c->path_cap;
 
     // One MLT path per slot [0, n): start, bounce loop.
     auto run_paths = [&](bool fresh, int id0, int n) {
-        hip_check(hipMemsetAsync(ctl, 0, sizeof(bnd::Ctl), s), "memset ctl");
+        bnd::launch_ctl_clear(ctl, s);
         { Timed tm(T, "mlt_start", s);
           bnd::launch_mlt_start(fresh, sc.dev, P, M, dp, ctl, (uint32_t*)c->queue[0].p, id0, n, gs, s); }
         int cur = 0;
@@ -446,13 +603,13 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
             { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
             { Timed tm(T, "trace_closest", s);
               bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
-                                        (uint32_t*)c->bins.p, bin_cap, (uint32_t*)c->spill.p, c->spill_stride,
-                                        (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt, s); }
+                                        (uint32_t*)c->spill.p, c->spill_stride, (uint32_t)c->refill,
+                                        (unsigned long long*)c->stats.p, gt, s); }
             { Timed tm(T, "bin", s);
               bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
             { Timed tm(T, "shade_mlt", s);
               bnd::launch_shade_mlt(fresh, sc.dev, P, M, ctl, (const uint32_t*)c->bins.p, bin_cap,
-                                    (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p, (const int4*)sc.lrec.p,
+                                    (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p,
                                     depth, p.max_depth, p.rr_depth, gs, s); }
             { Timed tm(T, "trace_shadow", s);
               bnd::launch_trace_shadow(false, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
@@ -530,7 +687,8 @@ int bn_struct_sizes(int32_t* out, int32_t n) {
                          (int32_t)sizeof(bn_camera), (int32_t)sizeof(bn_scene_desc), (int32_t)sizeof(bn_pt_params),
                          (int32_t)sizeof(bn_mlt_params), (int32_t)sizeof(bn_mlt_stats),
                          (int32_t)sizeof(bn_trace_stats), (int32_t)sizeof(bn_bvh_node),
-                         (int32_t)sizeof(bn_instance_upload)};
+                         (int32_t)sizeof(bn_instance_upload), (int32_t)sizeof(bn_batch_item),
+                         (int32_t)sizeof(bn_batch_stats), (int32_t)sizeof(bn_kernel_time)};
     const int m = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
     return m;
@@ -795,6 +953,8 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "mlt_chunk") {
             if (value < 64 || value > ((int64_t)1 << 26)) throw Error(BN_E_INVALID, "mlt_chunk out of range");
             c->mlt_chunk = (size_t)value;
+        } else if (k == "graph") {
+            c->use_graph = value != 0;
         } else if (k == "refill") {
             if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
             c->refill = (int)value;
@@ -852,5 +1012,15 @@ extern "C" int bn_pssmlt_last(bn_ctx* ctx, float* bootstrap, int32_t n_bootstrap
             if (n_chains != (int32_t)c->last_chain_accepts.size()) throw Error(BN_E_INVALID, "chain count mismatch");
             std::memcpy(chain_accepts, c->last_chain_accepts.data(), c->last_chain_accepts.size() * 4);
         }
+    });
+}
+
+extern "C" int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* const* films,
+                               bn_batch_stats* stats) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c) throw Error(BN_E_INVALID, "null context");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        render_batch(c, items, n, films, stats);
     });
 }

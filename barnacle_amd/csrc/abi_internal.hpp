@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <string>
 #include <utility>
@@ -60,6 +61,14 @@ struct KernelTimer {
     ~KernelTimer();
 };
 
+struct GraphExec {
+    hipGraphExec_t e = nullptr;
+    explicit GraphExec(hipGraphExec_t x) : e(x) {}
+    GraphExec(GraphExec&& o) noexcept : e(o.e) { o.e = nullptr; }
+    GraphExec(const GraphExec&) = delete;
+    ~GraphExec() { if (e) (void)hipGraphExecDestroy(e); }
+};
+
 struct Ctx {
     int device = 0;
     int n_cu = 1;
@@ -71,11 +80,15 @@ struct Ctx {
     std::vector<std::unique_ptr<SceneRec>> scenes;
     // path state
     size_t path_cap = 0;
-    DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b;
+    DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b, p_item;
     DevBuf queue[2], shq, bins, spill, film, ctl, stats;
     size_t spill_stride = 0;
     DevBuf out_a, out_b, out_c;
     KernelTimer timer;
+    // batched rounds
+    bool use_graph = true;
+    DevBuf b_scenes, b_items, b_hdr;
+    std::map<std::string, GraphExec> graphs;   // captured bounce loops by configuration
     // PSSMLT chain state
     size_t mlt_cap = 0;
     int mlt_nx = 0;

@@ -125,7 +125,7 @@ struct DevScene {
     const DevInst* insts;
     const float4* mats;         // 2 per material: (kind bits, albedo) (ior, alpha, metallic, 0)
     const float4* lightdefs;    // (emission, two_sided)
-    const int* light_insts;
+    const int4* lrec;           // per light: instance, first triangle, triangle count, light definition
     int n_lights;
     const float* alias_prob;    // per triangle
     const int* alias_idx;

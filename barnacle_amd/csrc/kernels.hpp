@@ -19,11 +19,12 @@ struct PathBuf {
     float4* beta;
     float4* L;
     uint32_t* rng;    // Sampler state (Base/Sampler.fs)
-    int4* hit;        // instance, primitive, t bits
+    int4* hit;        // instance, primitive, t bits, material bin
     float4* sh_o;     // shadow ray origin, w = t max
     float4* sh_d;
     float4* nee_a;    // beta * f
     float4* nee_b;    // Le / (pdf_b + pdf_l)
+    int* item;        // render item of the slot (batched rounds)
 };
 
 // Device-side counters of one bounce loop.
@@ -35,31 +36,76 @@ struct Ctl {
     uint32_t pad[4];
 };
 
-struct ChunkInfo {
-    int width, height;
-    int tiles_x, tiles_y;
-    int p_pad;        // tiles_x * tiles_y * 64
-    int s0, ns;       // samples [s0, s0 + ns) of this chunk
-    int spp, frame_id;
+// One render item of a round: samples [s0, s0 + ns) of every pixel of one
+// film, in slots [slot_base, slot_base + ns * p_pad).
+struct RoundItem {
+    int slot_base, slot_count;
+    int scene;                // index into the scene table (batch)
+    int width, height, tiles_x, p_pad;
+    int s0, ns, spp, frame_id;
+    int max_depth, rr_depth;
+    float inv_spp;            // ReciprocalEstimate(spp)
+    int film_offset;          // float offset of the item's film
+    int pix_end;              // cumulative pixel count up to and including this item
 };
 
-void launch_raygen(const DevScene& S, const PathBuf& P, Ctl* ctl, uint32_t* queue, const ChunkInfo& c,
-                   int grid, hipStream_t st);
-void launch_reset(Ctl* ctl, int nxt, hipStream_t st);
-void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride, uint32_t refill,
-                          unsigned long long* stats, int grid, hipStream_t st);
+// Sizes of a batched round, read by the kernels from device memory so that a
+// captured bounce loop can be replayed for every round.
+struct RoundHeader {
+    int n_items;
+    uint32_t n_slots;
+    uint32_t n_pix;
+    int pad;
+};
+
+// Everything a bounce loop needs besides buffers.
+struct Launch {
+    DevScene scene;           // plain mode
+    RoundItem item;           // plain mode
+    const RoundHeader* hdr;   // batch mode (device)
+    const RoundItem* items;   // batch mode (device)
+    const DevScene* scenes;   // batch mode (device)
+    uint32_t n_pix;           // plain mode
+    int grid_trace, grid_shade;
+    uint32_t refill;
+    hipStream_t stream;
+};
+
+struct Buffers {
+    uint32_t* queue[2];
+    uint32_t* shq;
+    uint32_t* bins;
+    uint32_t bin_cap;
+    uint32_t* spill;
+    size_t spill_stride;
+    unsigned long long* stats;
+    float* film;
+};
+
+// Called around every launch of a bounce loop (timing, counters).
+struct LaunchHook {
+    virtual void begin(const char* name, hipStream_t st) = 0;
+    virtual void end(hipStream_t st) = 0;
+    virtual void after_bounce(int cur) = 0;
+    virtual ~LaunchHook() = default;
+};
+
+// ctl_clear, raygen, [reset, trace_closest, bin, shade, trace_shadow] x depth_n, film
+void enqueue_bounce_loop(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, int depth_n,
+                         bool count, LaunchHook* hook);
+// NormalIntegrator frame: raygen, closest hit, normal shading, film.
+void enqueue_normal(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B);
+
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
                 int grid, hipStream_t st);
-void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
-                  uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec, int depth, int max_depth,
-                  int rr_depth, int grid, hipStream_t st);
+void launch_reset(Ctl* ctl, int nxt, hipStream_t st);
+void launch_ctl_clear(Ctl* ctl, hipStream_t st);
+void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
+                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
+                          hipStream_t st);
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
                          hipStream_t st);
-void launch_film(const PathBuf& P, float* film, const ChunkInfo& c, float inv_spp, hipStream_t st);
-void launch_shade_normal(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
-                         int grid, hipStream_t st);
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st);
 

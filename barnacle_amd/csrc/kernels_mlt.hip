@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_mlt_start(DevScene S, PathBuf P, MltBuf
 template <bool FRESH>
 __global__ __launch_bounds__(256) void k_shade_mlt(DevScene S, PathBuf P, MltBuf M, Ctl* ctl, const uint32_t* bins,
                                                    uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                                                   const int4* lrec, int depth, int max_depth, int rr_depth) {
+                                                   int depth, int max_depth, int rr_depth) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_shade_mlt(DevScene S, PathBuf P, MltBuf
             if (info.z >= 0) {
                 const Mat m = load_mat(S, info.z);
                 const V3 wo_l = to_local(g, -r.d);
-                const LightSample ls = light_sample(S, lrec, g.p, uLight, uEmit);
+                const LightSample ls = light_sample(S, g.p, uLight, uEmit);
                 if (ls.pdf != 0.0f) {
                     const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
                     const float dist = length(ls.p - g.p);
@@ -363,13 +363,13 @@ void launch_mlt_start(bool fresh, const DevScene& S, const PathBuf& P, const Mlt
 }
 void launch_shade_mlt(bool fresh, const DevScene& S, const PathBuf& P, const MltBuf& M, Ctl* ctl,
                       const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                      const int4* lrec, int depth, int max_depth, int rr_depth, int grid, hipStream_t st) {
+                      int depth, int max_depth, int rr_depth, int grid, hipStream_t st) {
     if (fresh)
         hipLaunchKernelGGL(k_shade_mlt<true>, dim3(grid), dim3(256), 0, st, S, P, M, ctl, bins, bin_cap, queue_next,
-                           nxt, shq, lrec, depth, max_depth, rr_depth);
+                           nxt, shq, depth, max_depth, rr_depth);
     else
         hipLaunchKernelGGL(k_shade_mlt<false>, dim3(grid), dim3(256), 0, st, S, P, M, ctl, bins, bin_cap, queue_next,
-                           nxt, shq, lrec, depth, max_depth, rr_depth);
+                           nxt, shq, depth, max_depth, rr_depth);
 }
 void launch_mlt_boot_done(const PathBuf& P, float* weights, int id0, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_mlt_boot_done, dim3(blocks(n)), dim3(256), 0, st, P, weights, id0, n);

@@ -3,8 +3,7 @@
 // One pass of PathTracingIntegrator.Li (Extensions/Integrator/PathTracing.fs:14-81)
 // is split into per-bounce kernels over compacted queues of path slots:
 //
-//   raygen -> [ reset -> trace_closest (bins by material) -> shade
-//               -> trace_shadow ] x max_depth -> film
+//   raygen -> [ reset -> trace_closest -> bin -> shade -> trace_shadow ] x depth -> film
 //
 // A path's state lives in SoA arrays indexed by its slot.  All paths that are
 // alive at bounce k have depth k, so depth is the loop index.  The L update
@@ -13,6 +12,13 @@
 // (pixel, sample) is kept per slot and folded into the film in sample order by
 // k_film, which makes the film bit-identical to ProgressiveIntegrator.RenderTile
 // (Base/Integrator.fs:36-44).
+//
+// Every kernel exists in two modes.  Plain: one scene and one film (kernel
+// arguments).  Batch: the slots of one round belong to several render items
+// (scene, film, depth limits, camera) of a device-side round table; each slot
+// records its item, and each lane reads its own scene.  Per-path random
+// streams do not depend on scheduling, so an item's film is bit-identical in
+// both modes.
 #include "kernels.hpp"
 #include "shading.hpp"
 #include "traverse.hpp"
@@ -42,15 +48,52 @@ __device__ __forceinline__ void wave_push(bool pred, uint32_t* ctr, uint32_t* ar
     if (pred) arr[base + mbcnt(m)] = val;
 }
 __device__ __forceinline__ void wave_add_stats(unsigned long long* dst, uint32_t v) {
-    // sum over the wave, one 64-bit atomic
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if (lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-// slot -> (pixel, sample) for a chunk: 8x8 pixel tiles, sample-major.
-__device__ __forceinline__ bool slot_pixel(const ChunkInfo& c, uint32_t slot, int& x, int& y, int& s) {
-    const uint32_t q = slot % (uint32_t)c.p_pad;
-    s = c.s0 + (int)(slot / (uint32_t)c.p_pad);
+// ---------------------------------------------------------- work context
+// Plain mode: one item, one scene.  Batch mode: round table + scene table.
+template <bool BATCH>
+struct Work;
+
+template <>
+struct Work<false> {
+    DevScene S;
+    RoundItem it;
+    __device__ __forceinline__ int n() const { return 1; }
+    __device__ __forceinline__ int item_of_new_slot(uint32_t) const { return 0; }
+    __device__ __forceinline__ int item_of(const PathBuf&, uint32_t) const { return 0; }
+    __device__ __forceinline__ const RoundItem& item(int) const { return it; }
+    __device__ __forceinline__ const DevScene& scene(int) const { return S; }
+    __device__ __forceinline__ uint32_t total_slots() const { return (uint32_t)it.slot_count; }
+};
+
+template <>
+struct Work<true> {
+    const RoundHeader* hdr;
+    const RoundItem* items;
+    const DevScene* scenes;
+    __device__ __forceinline__ int n() const { return hdr->n_items; }
+    // item owning slot (the round table is ordered by slot_base)
+    __device__ __forceinline__ int item_of_new_slot(uint32_t slot) const {
+        int lo = 0, hi = hdr->n_items - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((uint32_t)items[mid].slot_base <= slot) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ int item_of(const PathBuf& P, uint32_t slot) const { return P.item[slot]; }
+    __device__ __forceinline__ const RoundItem& item(int k) const { return items[k]; }
+    __device__ __forceinline__ const DevScene& scene(int k) const { return scenes[items[k].scene]; }
+    __device__ __forceinline__ uint32_t total_slots() const { return hdr->n_slots; }
+};
+
+// local slot -> (pixel, sample): 8x8 pixel tiles, sample-major.
+__device__ __forceinline__ bool slot_pixel(const RoundItem& c, uint32_t q_all, int& x, int& y, int& s) {
+    const uint32_t q = q_all % (uint32_t)c.p_pad;
+    s = c.s0 + (int)(q_all / (uint32_t)c.p_pad);
     const uint32_t tile = q >> 6;
     x = (int)((tile % (uint32_t)c.tiles_x) * 8 + (q & 7));
     y = (int)((tile / (uint32_t)c.tiles_x) * 8 + ((q >> 3) & 7));
@@ -59,30 +102,42 @@ __device__ __forceinline__ bool slot_pixel(const ChunkInfo& c, uint32_t slot, in
 
 // ------------------------------------------------------------------ raygen
 // RenderTile seeding and primary ray (Base/Integrator.fs:34-39).
-__global__ __launch_bounds__(256) void k_raygen(DevScene S, PathBuf P, Ctl* ctl, uint32_t* queue, ChunkInfo c) {
-    const uint32_t n = (uint32_t)c.ns * (uint32_t)c.p_pad;
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_raygen(Work<BATCH> W, PathBuf P, Ctl* ctl, uint32_t* queue) {
+    const uint32_t n = W.total_slots();
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t slot = base + threadIdx.x;
         int x = 0, y = 0, s = 0;
-        bool ok = slot < n && slot_pixel(c, slot, x, y, s);
-        if (ok) {
-            uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(c.frame_id * c.spp + s));
-            V2 uPix, uLens;
-            uPix.x = lcg(rng); uPix.y = lcg(rng);
-            uLens.x = lcg(rng); uLens.y = lcg(rng);
-            Ray r = camera_ray(S, x, y, uPix, uLens);
-            P.o[slot] = f4(r.o, 0.0f);
-            P.d[slot] = f4(r.d, 0.0f);
-            P.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            P.rng[slot] = rng;
+        bool ok = false;
+        if (slot < n) {
+            const int k = W.item_of_new_slot(slot);
+            const RoundItem& it = W.item(k);
+            ok = slot_pixel(it, slot - (uint32_t)it.slot_base, x, y, s);
+            if (ok) {
+                const DevScene& S = W.scene(k);
+                uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(it.frame_id * it.spp + s));
+                V2 uPix, uLens;
+                uPix.x = lcg(rng); uPix.y = lcg(rng);
+                uLens.x = lcg(rng); uLens.y = lcg(rng);
+                Ray r = camera_ray(S, x, y, uPix, uLens);
+                P.o[slot] = f4(r.o, 0.0f);
+                P.d[slot] = f4(r.d, 0.0f);
+                P.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                P.rng[slot] = rng;
+                if (BATCH) P.item[slot] = k;
+            }
         }
         wave_push(ok, &ctl->q_count[0], queue, slot);
     }
 }
 
-// Per-bounce counter reset (kept as a kernel so the bounce loop can be
-// captured in a graph without memset nodes).
+// Counter clear at the start of a bounce loop, and per-bounce reset (kernels,
+// so that a captured graph carries no memset nodes).
+__global__ void k_ctl_clear(Ctl* ctl) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(ctl);
+    if (threadIdx.x < sizeof(Ctl) / 4) w[threadIdx.x] = 0;
+}
 __global__ void k_reset(Ctl* ctl, int nxt) {
     const int i = threadIdx.x;
     if (i < kBins) ctl->bin_count[i] = 0;
@@ -95,9 +150,9 @@ __global__ void k_reset(Ctl* ctl, int nxt) {
 // advances each by one item per iteration and refills idle lanes from the
 // queue once at least `refill` of them are idle (one atomic per wave).
 template <bool ANY, bool COUNT, typename Init, typename Done>
-__device__ __forceinline__ void persistent_trace(const DevScene& S, uint32_t* fetch_ctr, uint32_t count,
-                                                 uint32_t* stk, uint32_t* spill, size_t spill_stride,
-                                                 uint32_t refill, TraceCount& tc, Init init, Done done) {
+__device__ __forceinline__ void persistent_trace(uint32_t* fetch_ctr, uint32_t count, uint32_t* stk, uint32_t* spill,
+                                                 size_t spill_stride, uint32_t refill, TraceCount& tc, Init init,
+                                                 Done done) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     Traversal<kStackK> tr;
     tr.st = LaneStack<kStackK>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
@@ -123,7 +178,7 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, uint32_t* fe
             continue;
         }
         bool fin = false, found = false;
-        if (active) fin = !tr.template step<ANY, COUNT>(S, tc, found);
+        if (active) fin = !tr.template step<ANY, COUNT>(tc, found);
         if (__ballot(fin) != 0) {
             if (fin) active = false;
             done(tr, item, fin, found);
@@ -131,26 +186,28 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, uint32_t* fe
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_closest(DevScene S, PathBuf P, Ctl* ctl, int cur,
-                                                       const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
-                                                       unsigned long long* stats) {
+// BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  The
+// material bin of the hit (kind of the instance's material; light-only
+// instances in the last bin, -1 for a miss) travels in the hit record.
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
+                                                       const uint32_t* queue, uint32_t* spill, size_t spill_stride,
+                                                       uint32_t refill, unsigned long long* stats) {
     __shared__ uint32_t stk[kStackK * 256];
     const uint32_t count = ctl->q_count[cur];
     TraceCount tc{0, 0, 0};
     persistent_trace<false, COUNT>(
-        S, &ctl->fetch[0], count, stk, spill, spill_stride, refill, tc,
+        &ctl->fetch[0], count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<kStackK>& tr, uint32_t i) {
             const uint32_t slot = queue[i];
-            tr.start({xyz(P.o[slot]), xyz(P.d[slot])}, kInf, S.exact_only != 0);
+            tr.start(W.scene(W.item_of(P, slot)), {xyz(P.o[slot]), xyz(P.d[slot])}, kInf);
         },
         [&](Traversal<kStackK>& tr, uint32_t i, bool fin, bool) {
-            // the material bin travels with the hit; k_bin compacts the bins
             if (fin) {
                 const uint32_t slot = queue[i];
                 int bin = -1;
                 if (tr.hit_inst >= 0) {
+                    const DevScene& S = W.scene(W.item_of(P, slot));
                     const int4 info = S.insts[tr.hit_inst].info;
                     bin = info.z >= 0 ? __float_as_int(S.mats[2 * info.z].x) : kBins - 1;
                 }
@@ -168,7 +225,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(DevScene S, PathBuf P, Ct
 // kind).  A block bins kBinItems consecutive queue entries: positions inside
 // the block come from wave-aggregated LDS counters, and each bin costs one
 // global atomic per block.
-constexpr int kBinPer = 16;                       // entries per thread
+constexpr int kBinPer = 16;
 constexpr int kBinItems = 256 * kBinPer;
 
 __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins,
@@ -210,12 +267,13 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
 
 // ------------------------------------------------------------------- shade
 // The body of one loop iteration of PathTracing.fs:24-79 after the closest
-// hit: emission with MIS (:30-40), next-event estimation setup (:42-50), BSDF
+// hit: emission with MIS (:30-40), next-event estimation set-up (:42-50), BSDF
 // sampling (:61-67) and Russian roulette (:69-75).  The NEE contribution is
 // applied by k_trace_shadow when the shadow ray is unoccluded (:47-59).
-__global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
-                                               uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec,
-                                               int depth, int max_depth, int rr_depth) {
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* bins,
+                                               uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
+                                               int depth) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
@@ -228,6 +286,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
             int b = 0;
             while (i >= start[b + 1]) ++b;
             slot = bins[(size_t)b * bin_cap + (i - start[b])];
+            const int k = W.item_of(P, slot);
+            const DevScene& S = W.scene(k);
+            const RoundItem& it = W.item(k);
             const float4 o4 = P.o[slot], d4 = P.d[slot];
             const Ray r = {xyz(o4), xyz(d4)};
             const float prev_pdf = d4.w;
@@ -251,7 +312,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
                 const float uSel = lcg(rng);
                 V2 uL;
                 uL.x = lcg(rng); uL.y = lcg(rng);
-                const LightSample ls = light_sample(S, lrec, g.p, uSel, uL);
+                const LightSample ls = light_sample(S, g.p, uSel, uL);
                 if (ls.pdf != 0.0f) {
                     const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
                     const float dist = length(ls.p - g.p);
@@ -270,7 +331,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
                     const V3 wi = to_world(g, wi_l);
                     beta = beta * bs.f * rcp_e(bs.pdf);
                     cont = true;
-                    if (depth >= rr_depth) {
+                    if (depth >= it.rr_depth) {
                         const float q = min_n(1.0f, max_n(beta.x, max_n(beta.y, beta.z)));
                         if (lcg(rng) < q) beta = beta * rcp_e(q);
                         else cont = false;
@@ -280,7 +341,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
                     P.beta[slot] = f4(beta, 0.0f);
                 }
                 P.rng[slot] = rng;
-                cont = cont && depth + 1 < max_depth;
+                cont = cont && depth + 1 < it.max_depth;
             }
             P.L[slot] = f4(L, 0.0f);
         }
@@ -292,19 +353,19 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, PathBuf P, Ctl* ctl, 
 // ----------------------------------------------------------- trace shadow
 // aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
 // an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_shadow(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* shq,
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats) {
     __shared__ uint32_t stk[kStackK * 256];
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT>(
-        S, &ctl->fetch[1], count, stk, spill, spill_stride, refill, tc,
+        &ctl->fetch[1], count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<kStackK>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
-            tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, S.exact_only != 0);
+            tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
             // a NaN t max fails every slab test of the reference: empty traversal
             if (!(o4.w == o4.w)) tr.st.sp = 0;
         },
@@ -324,26 +385,43 @@ __global__ __launch_bounds__(256) void k_trace_shadow(DevScene S, PathBuf P, Ctl
 
 // -------------------------------------------------------------------- film
 // accum = FMA(1/spp, Li / pdf, accum) in sample order (Integrator.fs:41-42);
-// the camera pdf is 1.  The film keeps Film.Pixels order (Film.fs:43).
-__global__ void k_film(PathBuf P, float* film, ChunkInfo c, float inv_spp) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= c.width || y >= c.height) return;
-    const uint32_t q = ((uint32_t)((y >> 3) * c.tiles_x + (x >> 3)) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
-    const size_t fi = 3 * ((size_t)(c.height - y - 1) * c.width + x);
-    float ax = film[fi], ay = film[fi + 1], az = film[fi + 2];
-    for (int s = 0; s < c.ns; ++s) {
-        const float4 L = P.L[(size_t)s * c.p_pad + q];
-        ax = fmaf(inv_spp, L.x * rcp_e(1.0f), ax);
-        ay = fmaf(inv_spp, L.y * rcp_e(1.0f), ay);
-        az = fmaf(inv_spp, L.z * rcp_e(1.0f), az);
+// the camera pdf is 1.  The film keeps Film.Pixels order (Film.fs:43).  One
+// thread handles one pixel of one item of the round (items are ordered by
+// their cumulative pixel end pix_end).
+template <bool BATCH>
+__global__ void k_film(Work<BATCH> W, PathBuf P, float* film) {
+    const int n_items = W.n();
+    const uint32_t n_pix = (uint32_t)W.item(n_items - 1).pix_end;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += gridDim.x * blockDim.x) {
+        int k = 0;
+        if (BATCH) {
+            int lo = 0, hi = n_items - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((uint32_t)W.item(mid).pix_end <= p) lo = mid + 1; else hi = mid;
+            }
+            k = lo;
+        }
+        const RoundItem& c = W.item(k);
+        const uint32_t lp = p - (uint32_t)(c.pix_end - c.width * c.height);
+        const int x = (int)(lp % (uint32_t)c.width), y = (int)(lp / (uint32_t)c.width);
+        const uint32_t q =
+            ((uint32_t)((y >> 3) * c.tiles_x + (x >> 3)) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
+        float* f = film + c.film_offset + 3 * ((size_t)(c.height - y - 1) * c.width + x);
+        float ax = f[0], ay = f[1], az = f[2];
+        for (int s = 0; s < c.ns; ++s) {
+            const float4 L = P.L[(size_t)c.slot_base + (size_t)s * c.p_pad + q];
+            ax = fmaf(c.inv_spp, L.x * rcp_e(1.0f), ax);
+            ay = fmaf(c.inv_spp, L.y * rcp_e(1.0f), ay);
+            az = fmaf(c.inv_spp, L.z * rcp_e(1.0f), az);
+        }
+        f[0] = ax; f[1] = ay; f[2] = az;
     }
-    film[fi] = ax; film[fi + 1] = ay; film[fi + 2] = az;
 }
 
 // NormalIntegrator.Li (Extensions/Integrator/Normal.fs:10-17) on the bins of
 // the first bounce: 0.5 * (n + 1).
-__global__ void k_shade_normal(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap) {
+__global__ void k_shade_normal(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
@@ -354,7 +432,7 @@ __global__ void k_shade_normal(DevScene S, PathBuf P, Ctl* ctl, const uint32_t* 
         const uint32_t slot = bins[(size_t)b * bin_cap + (i - start[b])];
         const Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
         const int4 h = P.hit[slot];
-        const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+        const Geom g = hit_geometry(W.S, r, h.x, h.y, __int_as_float(h.z));
         P.L[slot] = f4(0.5f * (g.n + mk(1, 1, 1)), 0.0f);
     }
 }
@@ -367,14 +445,14 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
     const uint32_t n = (uint32_t)width * (uint32_t)height;
     TraceCount tc{0, 0, 0};
     persistent_trace<false, false>(
-        S, fetch, n, stk, spill, spill_stride, 16u, tc,
+        fetch, n, stk, spill, spill_stride, 16u, tc,
         [&](Traversal<kStackK>& tr, uint32_t p) {
             const int x = (int)(p % (uint32_t)width), y = (int)(p / (uint32_t)width);
             uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
             V2 uPix, uLens;
             uPix.x = lcg(rng); uPix.y = lcg(rng);
             uLens.x = lcg(rng); uLens.y = lcg(rng);
-            tr.start(camera_ray(S, x, y, uPix, uLens), kInf, S.exact_only != 0);
+            tr.start(S, camera_ray(S, x, y, uPix, uLens), kInf);
         },
         [&](Traversal<kStackK>& tr, uint32_t p, bool fin, bool) {
             if (fin) {
@@ -386,49 +464,120 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
 }
 
 // ------------------------------------------------------------------ launch
-void launch_raygen(const DevScene& S, const PathBuf& P, Ctl* ctl, uint32_t* queue, const ChunkInfo& c,
-                   int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, st, S, P, ctl, queue, c);
+static Work<false> plain_work(const Launch& L) {
+    Work<false> w;
+    w.S = L.scene;
+    w.it = L.item;
+    return w;
 }
-void launch_reset(Ctl* ctl, int nxt, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt);
+static Work<true> batch_work(const Launch& L) {
+    Work<true> w;
+    w.hdr = L.hdr;
+    w.items = L.items;
+    w.scenes = L.scenes;
+    return w;
 }
-void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* bins, uint32_t bin_cap, uint32_t* spill, size_t spill_stride, uint32_t refill,
-                          unsigned long long* stats, int grid, hipStream_t st) {
-    if (count)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
-                           spill, spill_stride, refill, stats);
-    else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, cur, queue, bins, bin_cap,
-                           spill, spill_stride, refill, stats);
+
+template <bool BATCH>
+static void enqueue_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                      int depth_n, bool count, LaunchHook* hook) {
+    hipStream_t st = L.stream;
+    struct Bracket {
+        LaunchHook* h;
+        hipStream_t s;
+        Bracket(LaunchHook* hk, const char* name, hipStream_t st_) : h(hk), s(st_) { if (h) h->begin(name, s); }
+        ~Bracket() { if (h) h->end(s); }
+    };
+    { Bracket b(hook, "ctl_clear", st); hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl); }
+    { Bracket b(hook, "raygen", st);
+      hipLaunchKernelGGL(k_raygen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.queue[0]); }
+    int cur = 0;
+    for (int depth = 0; depth < depth_n; ++depth) {
+        const int nxt = 1 - cur;
+        { Bracket b(hook, "reset", st); hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt); }
+        { Bracket b(hook, "trace_closest", st);
+          if (count)
+              hipLaunchKernelGGL((k_trace_closest<true, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl, cur,
+                                 B.queue[cur], B.spill, B.spill_stride, L.refill, B.stats);
+          else
+              hipLaunchKernelGGL((k_trace_closest<false, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
+                                 cur, B.queue[cur], B.spill, B.spill_stride, L.refill, B.stats); }
+        { Bracket b(hook, "bin", st);
+          hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, cur, B.queue[cur], B.bins,
+                             B.bin_cap); }
+        { Bracket b(hook, "shade", st);
+          hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
+                             B.queue[nxt], nxt, B.shq, depth); }
+        { Bracket b(hook, "trace_shadow", st);
+          if (count)
+              hipLaunchKernelGGL((k_trace_shadow<true, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
+                                 B.shq, B.spill, B.spill_stride, L.refill, B.stats);
+          else
+              hipLaunchKernelGGL((k_trace_shadow<false, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
+                                 B.shq, B.spill, B.spill_stride, L.refill, B.stats); }
+        if (hook) hook->after_bounce(cur);
+        cur = nxt;
+    }
+    { Bracket b(hook, "film", st);
+      hipLaunchKernelGGL(k_film<BATCH>, dim3(BATCH ? L.grid_shade : (L.n_pix + 255) / 256), dim3(256), 0, st, W, P,
+                         B.film); }
 }
+
+void enqueue_bounce_loop(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, int depth_n,
+                         bool count, LaunchHook* hook) {
+    if (batch) enqueue_t<true>(batch_work(L), L, P, ctl, B, depth_n, count, hook);
+    else enqueue_t<false>(plain_work(L), L, P, ctl, B, depth_n, count, hook);
+}
+
+void enqueue_normal(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B) {
+    const Work<false> W = plain_work(L);
+    hipStream_t st = L.stream;
+    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
+    hipLaunchKernelGGL(k_raygen<false>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.queue[0]);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, 1);
+    hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl, 0,
+                       B.queue[0], B.spill, B.spill_stride, L.refill, B.stats);
+    hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, 0, B.queue[0], B.bins, B.bin_cap);
+    hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap);
+    hipLaunchKernelGGL(k_film<false>, dim3((L.n_pix + 255) / 256), dim3(256), 0, st, W, P, B.film);
+}
+
+// Single kernels used by the PSSMLT driver (plain scene).
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
                 int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap);
 }
-void launch_shade(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
-                  uint32_t* queue_next, int nxt, uint32_t* shq, const int4* lrec, int depth, int max_depth,
-                  int rr_depth, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, S, P, ctl, bins, bin_cap, queue_next, nxt, shq, lrec,
-                       depth, max_depth, rr_depth);
+void launch_reset(Ctl* ctl, int nxt, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt);
+}
+void launch_ctl_clear(Ctl* ctl, hipStream_t st) {
+    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
+}
+void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
+                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
+                          hipStream_t st) {
+    Work<false> W;
+    W.S = S;
+    W.it = RoundItem{};
+    if (count)
+        hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, spill,
+                           spill_stride, refill, stats);
+    else
+        hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, spill,
+                           spill_stride, refill, stats);
 }
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
                          hipStream_t st) {
+    Work<false> W;
+    W.S = S;
+    W.it = RoundItem{};
     if (count)
-        hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
-                           refill, stats);
+        hipLaunchKernelGGL((k_trace_shadow<true, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, spill,
+                           spill_stride, refill, stats);
     else
-        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid), dim3(256), 0, st, S, P, ctl, shq, spill, spill_stride,
-                           refill, stats);
-}
-void launch_film(const PathBuf& P, float* film, const ChunkInfo& c, float inv_spp, hipStream_t st) {
-    hipLaunchKernelGGL(k_film, dim3((c.width + 255) / 256, c.height), dim3(256), 0, st, P, film, c, inv_spp);
-}
-void launch_shade_normal(const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
-                         int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade_normal, dim3(grid), dim3(256), 0, st, S, P, ctl, bins, bin_cap);
+        hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, spill,
+                           spill_stride, refill, stats);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {

@@ -153,13 +153,13 @@ struct LightSample { V3 p, L; float pdf; V3 wi; };
 // UniformLightSampler.Sample (Uniform.fs:13-29) over MeshInstance.Sample
 // (Mesh.fs:289-298, AliasTable.fs:52-62, Triangle.Sample Mesh.fs:89-99) or
 // SphereInstance.Sample (Sphere.fs:90-113).
-__device__ __forceinline__ LightSample light_sample(const DevScene& S, const int4* lrec, V3 p, float uSel, V2 u) {
+__device__ __forceinline__ LightSample light_sample(const DevScene& S, V3 p, float uSel, V2 u) {
     const int N = S.n_lights;
     float us = uSel * (float)N;
     int id = sat_int(us);
     id = id < N - 1 ? id : N - 1;
     us = us - (float)id;
-    const int4 rec = lrec[id];
+    const int4 rec = S.lrec[id];
     const DevInst& I = S.insts[rec.x];
     const Xf o2w = I.o2w;
     V3 lp, ln;

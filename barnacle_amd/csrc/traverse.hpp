@@ -37,8 +37,18 @@ __device__ __forceinline__ bool finite3(V3 v) {
 
 enum : int { kItemNone = 0, kItemTNode = 1, kItemBNode = 2, kItemTri = 3, kItemInst = 4 };
 
+// The geometry a lane traverses (per lane, so a batched launch can mix scenes).
+struct TraceScene {
+    const float4* tlas;
+    const float4* blas;
+    const float* tris;
+    const DevInst* insts;
+    __device__ __forceinline__ void set(const DevScene& S) { tlas = S.tlas; blas = S.blas; tris = S.tris; insts = S.insts; }
+};
+
 template <int K>
 struct Traversal {
+    TraceScene G;
     // world ray and the ray of the current space (world, or object inside a BLAS)
     V3 wo, wd, winv;
     V3 co, cd, cinv;
@@ -52,7 +62,9 @@ struct Traversal {
     int bl_cur, bl_end;  // triangles left in the current BLAS leaf (global indices)
     int blas_root, tri_base, cur_inst;
 
-    __device__ __forceinline__ void start(const Ray& r, float tmax, bool exact_scene) {
+    __device__ __forceinline__ void start(const DevScene& S, const Ray& r, float tmax) {
+        G.set(S);
+        const bool exact_scene = S.exact_only != 0;
         wo = r.o; wd = r.d;
         winv = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
         co = wo; cd = wd; cinv = winv;
@@ -74,7 +86,7 @@ struct Traversal {
     // One item.  Returns false when the traversal has finished (no items left,
     // or an any-hit query found its hit); `found` reports an any-hit.
     template <bool ANY, bool COUNT>
-    __device__ __forceinline__ bool step(const DevScene& S, TraceCount& cnt, bool& found) {
+    __device__ __forceinline__ bool step(TraceCount& cnt, bool& found) {
         // leave a finished BLAS
         if (blas_base >= 0 && bl_cur >= bl_end && st.sp == blas_base) {
             blas_base = -1;
@@ -96,15 +108,15 @@ struct Traversal {
         Tri tr;
         NodeView n;
         if (kind == kItemTri) {
-            tr = load_tri(S.tris, idx);
+            tr = load_tri(G.tris, idx);
             lo = vmin(vmin(tr.p0, tr.p1), tr.p2);
             hi = vmax(vmax(tr.p0, tr.p1), tr.p2);
             if (COUNT) cnt.tris++;
         } else if (kind == kItemInst) {
-            const float4 a = S.insts[idx].lo, b = S.insts[idx].hi;
+            const float4 a = G.insts[idx].lo, b = G.insts[idx].hi;
             lo = xyz(a); hi = xyz(b);
         } else {
-            n = load_node(kind == kItemTNode ? S.tlas : S.blas + 2 * (size_t)blas_root, idx);
+            n = load_node(kind == kItemTNode ? G.tlas : G.blas + 2 * (size_t)blas_root, idx);
             lo = n.lo; hi = n.hi;
             if (COUNT) cnt.nodes++;
         }
@@ -119,7 +131,7 @@ struct Traversal {
             }
         } else if (kind == kItemInst) {
             if (COUNT) cnt.insts++;
-            const DevInst& I = S.insts[idx];
+            const DevInst& I = G.insts[idx];
             const Xf w2o = I.w2o;
             const V3 oo = xf_point(w2o, wo), od = xf_dir(w2o, wd);
             const int4 info = I.info;

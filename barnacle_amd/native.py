@@ -55,6 +55,8 @@ _SIGS = {
     "bn_reset_kernel_times": ([C.c_void_p], C.c_int),
     "bn_render_pssmlt": ([C.c_void_p, C.c_int32, C.POINTER(abi.MLTParams), f32p, C.POINTER(abi.MLTStats)],
                          C.c_int),
+    "bn_render_batch": ([C.c_void_p, C.POINTER(abi.BatchItem), C.c_int32, C.POINTER(f32p),
+                         C.POINTER(abi.BatchStats)], C.c_int),
     "bn_pssmlt_last": ([C.c_void_p, f32p, C.c_int32, i32p, C.c_int32], C.c_int),
 }
 
@@ -132,6 +134,20 @@ class Context:
 
     def scene(self, sc) -> "DeviceScene":
         return DeviceScene(self, sc)
+
+    def render_batch(self, jobs):
+        """jobs: list of (DeviceScene, PTParams).  One batched launch sequence;
+        returns (films, {"rounds", "graph"})."""
+        n = len(jobs)
+        items = (abi.BatchItem * n)()
+        films = [np.zeros((p.width * p.height, 3), np.float32) for _, p in jobs]
+        ptrs = (f32p * n)()
+        for k, (ds, p) in enumerate(jobs):
+            items[k] = abi.BatchItem(ds.id, p)
+            ptrs[k] = _p(films[k], f32p)
+        st = abi.BatchStats()
+        check(self.lib.bn_render_batch(self.h, items, n, ptrs, C.byref(st)))
+        return films, {"rounds": st.rounds, "graph": st.graph}
 
 
 class DeviceScene:

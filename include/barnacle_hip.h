@@ -268,13 +268,32 @@ int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
 int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
                  int32_t sample_id, int32_t* inst, int32_t* prim, float* t);
 
+/* Batched rendering (no reference counterpart: the reference renders one
+ * scene per Scene.Render).  Each item is one PathTracingIntegrator frame of
+ * a scene from bn_scene_build / bn_upload_scene; all items advance through
+ * one launch sequence per round (scenes, resolutions, spp and depths may
+ * differ), with the bounce loop of a round captured once as a HIP graph and
+ * replayed.  films[k] receives item k's film exactly as bn_render_pt would
+ * write it (bit-identical); NULL entries are skipped. */
+typedef struct bn_batch_item {
+    int32_t      scene;
+    bn_pt_params params;
+} bn_batch_item;
+typedef struct bn_batch_stats {
+    int32_t rounds;
+    int32_t graph;                     /* 1 when the rounds replayed a graph */
+} bn_batch_stats;
+int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* const* films,
+                    bn_batch_stats* stats);
+
 /* Tunables and instrumentation (no reference counterpart).  Keys:
  *   "timing"               1: record a HIP event pair around every kernel
  *                          launch of the render calls, on the context's stream
  *   "chunk_slots"          path slots in flight per bounce loop (default 8M)
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
  *   "refill"               idle lanes of a wave that trigger a refill (1..64)
- *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass */
+ *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
+ *   "graph"                1 (default): replay batched rounds as a HIP graph */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {
