@@ -103,9 +103,16 @@ class BatchStats(C.Structure):
     _fields_ = [("rounds", i32), ("graph", i32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("trace_blocks_per_cu", i32), ("refill", i32), ("stack_k", i32), ("trials", i32),
+                ("chunk_slots", i64), ("baseline_ms", C.c_float), ("best_ms", C.c_float)]
+
+
+BN_TUNE_BATCH = 1
+
 # Declaration order of include/barnacle_hip.h, as bn_struct_sizes() reports it.
 STRUCT_ORDER = [Primitive, Material, Light, Instance, Transform, Node, Camera,
                 SceneDesc, PTParams, MLTParams, MLTStats, TraceStats, BVHNode,
-                InstanceUpload, BatchItem, BatchStats, KernelTime]
+                InstanceUpload, BatchItem, BatchStats, KernelTime, TuneResult]
 
 BN_OK, BN_E_INVALID, BN_E_NO_LIGHT, BN_E_DEVICE, BN_E_UNSUPPORTED, BN_E_CAPACITY = 0, -1, -2, -3, -4, -5

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -303,7 +304,7 @@ bnd::PathBuf path_buf(Ctx* c) {
 
 void ensure_spill(Ctx* c) {
     const size_t lanes = (size_t)grid_trace(c) * bnd::kBlock;
-    const size_t need = lanes * (bnd::kStackMax - bnd::kStackK) * 4;
+    const size_t need = lanes * (bnd::kStackMax - bnd::kStackKMin) * 4;
     if (need > c->spill.bytes) c->graphs.clear();
     c->spill.ensure(need);
     c->spill_stride = lanes;
@@ -318,6 +319,7 @@ bnd::Buffers buffers(Ctx* c) {
     B.bin_cap = (uint32_t)c->path_cap;
     B.spill = (uint32_t*)c->spill.p;
     B.spill_stride = c->spill_stride;
+    B.stack_k = c->stack_k;
     B.stats = (unsigned long long*)c->stats.p;
     B.film = (float*)c->film.p;
     return B;
@@ -422,7 +424,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     hipStream_t s = c->stream;
     std::vector<bnd::RoundItem> base(n);
     std::vector<bnd::DevScene> scenes(n);
-    size_t film_total = 0;
+    size_t film_total = 0, slots_total = 0;
     int depth_n = 0;
     size_t max_pad = 0;
     for (int k = 0; k < n; ++k) {
@@ -439,8 +441,9 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         if (film_total > ((size_t)1 << 31)) throw Error(BN_E_CAPACITY, "batch films exceed 2^31 floats");
         depth_n = std::max(depth_n, p.max_depth);
         max_pad = std::max(max_pad, (size_t)base[k].p_pad);
+        slots_total += (size_t)base[k].p_pad * (size_t)std::max(p.spp, 0);
     }
-    const size_t cap = std::max(max_pad, c->chunk_slots);
+    const size_t cap = std::max(max_pad, std::min(c->chunk_slots, slots_total));
     ensure_paths(c, cap);
     ensure_spill(c);
     c->film.ensure(film_total * 4);
@@ -464,7 +467,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     hipGraphExec_t exec = nullptr;
     if (c->use_graph) {
         const std::string key = "batch:" + std::to_string(depth_n) + ":" + std::to_string(L.grid_trace) + ":" +
-                                std::to_string(L.refill) + ":" + std::to_string((uintptr_t)c->b_items.p) + ":" +
+                                std::to_string(L.refill) + ":" + std::to_string(B.stack_k) + ":" + std::to_string((uintptr_t)c->b_items.p) + ":" +
                                 std::to_string((uintptr_t)c->b_scenes.p) + ":" + std::to_string((uintptr_t)c->film.p);
         auto f = c->graphs.find(key);
         if (f == c->graphs.end()) {
@@ -488,21 +491,33 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         int active = 0;
         for (int k = 0; k < n; ++k) active += next[k] < items[k].params.spp;
         if (active == 0) break;
-        const size_t share = std::max<size_t>(1, cap / active);
-        // pass 1: a fair share for every unfinished item; pass 2: fill up
+        // Item-major packing keeps each round on few scenes (their BVHs stay
+        // in cache); fair-share packing gives every unfinished item a slice.
         std::vector<int> take(n, 0);
-        for (int pass = 0; pass < 2; ++pass)
+        if (c->batch_pack == 0) {
             for (int k = 0; k < n; ++k) {
-                const int left = items[k].params.spp - next[k] - take[k];
+                const int left = items[k].params.spp - next[k];
                 if (left <= 0) continue;
-                const size_t pad = (size_t)base[k].p_pad;
-                size_t fit = (cap - used) / pad;
-                if (pass == 0) fit = std::min(fit, std::max<size_t>(1, share / pad));
-                const int t = (int)std::min<size_t>((size_t)left, fit);
-                if (t <= 0) continue;
-                take[k] += t;
-                used += (size_t)t * pad;
+                const int t = (int)std::min<size_t>((size_t)left, (cap - used) / (size_t)base[k].p_pad);
+                if (t <= 0) break;
+                take[k] = t;
+                used += (size_t)t * base[k].p_pad;
             }
+        } else {
+            const size_t share = std::max<size_t>(1, cap / active);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < n; ++k) {
+                    const int left = items[k].params.spp - next[k] - take[k];
+                    if (left <= 0) continue;
+                    const size_t pad = (size_t)base[k].p_pad;
+                    size_t fit = (cap - used) / pad;
+                    if (pass == 0) fit = std::min(fit, std::max<size_t>(1, share / pad));
+                    const int t = (int)std::min<size_t>((size_t)left, fit);
+                    if (t <= 0) continue;
+                    take[k] += t;
+                    used += (size_t)t * pad;
+                }
+        }
         int slot = 0, pix = 0;
         for (int k = 0; k < n; ++k) {
             if (take[k] == 0) continue;
@@ -548,6 +563,91 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     }
 }
 
+// Launch-parameter autotuner: coordinate descent over the persistent trace
+// grid (blocks per CU), the lane-refill threshold, the LDS stack window and
+// the path slots in flight, timing the caller's own workload (plain renders
+// of every item in turn, or one batched render).  Each candidate runs once to
+// warm up (allocation, graph capture) and once timed; the best setting stays
+// on the context.  Results do not depend on any of these parameters.
+void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_result* out) {
+    if (!items || n <= 0) throw Error(BN_E_INVALID, "empty workload");
+    const bool batched = (flags & BN_TUNE_BATCH) != 0;
+    size_t slots_total = 0;
+    for (int k = 0; k < n; ++k) {
+        check_pt_params(items[k].params);
+        const bnd::RoundItem it = plain_item(items[k].params);
+        slots_total += (size_t)it.p_pad * (size_t)std::max(items[k].params.spp, 0);
+    }
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    // path state + queues + bins per slot
+    const size_t per_slot = 8 * sizeof(float4) + sizeof(int4) + 4 * 5 + 4 * bnd::kBins;
+    const size_t slot_limit = (free_b + c->path_cap * per_slot) / 4 / per_slot;
+
+    auto run = [&]() -> double {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (batched) {
+            render_batch(c, items, n, nullptr, nullptr);
+        } else {
+            for (int k = 0; k < n; ++k)
+                render_frames(c, get_scene(c, items[k].scene), items[k].params, false, nullptr, nullptr);
+        }
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto measure = [&]() { run(); return run(); };
+
+    struct Knob { int64_t* cur; std::vector<int64_t> values; };
+    int64_t blocks = c->trace_blocks_per_cu, refill = c->refill, stack_k = c->stack_k;
+    int64_t chunk = (int64_t)c->chunk_slots;
+    auto apply = [&]() {
+        c->trace_blocks_per_cu = (int)blocks;
+        c->refill = (int)refill;
+        c->stack_k = (int)stack_k;
+        c->chunk_slots = (size_t)chunk;
+    };
+    std::vector<int64_t> chunks;
+    for (int64_t v = (int64_t)1 << 22; v <= ((int64_t)1 << 26); v <<= 1) {
+        if ((size_t)v > slot_limit) break;
+        chunks.push_back(v);
+        if ((size_t)v >= slots_total) break;          // larger chunks change nothing
+    }
+    if (chunks.empty()) chunks.push_back(chunk);
+    std::vector<Knob> knobs = {{&blocks, {2, 3, 4, 5}}, {&refill, {32, 40, 48, 56}}, {&stack_k, {16, 32}},
+                               {&chunk, chunks}};
+    apply();
+    double best = measure();
+    const double baseline = best;
+    int trials = 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool changed = false;
+        for (Knob& kb : knobs) {
+            const int64_t keep = *kb.cur;
+            int64_t arg = keep;
+            for (int64_t v : kb.values) {
+                if (v == keep) continue;
+                *kb.cur = v;
+                apply();
+                const double t = measure();
+                ++trials;
+                if (t < best * 0.99) { best = t; arg = v; }
+            }
+            *kb.cur = arg;
+            apply();
+            changed |= arg != keep;
+        }
+        if (!changed) break;
+    }
+    if (out) {
+        out->trace_blocks_per_cu = (int32_t)blocks;
+        out->refill = (int32_t)refill;
+        out->stack_k = (int32_t)stack_k;
+        out->trials = trials;
+        out->chunk_slots = chunk;
+        out->baseline_ms = (float)baseline;
+        out->best_ms = (float)best;
+    }
+}
+
 // PSSMLTIntegrator.Render (Extensions/Integrator/PSSMLT.fs:379-414): the
 // bootstrap paths and the chains run as wavefronts on the device; the mean
 // bootstrap luminance B (Array.average, sequential) and the bootstrap alias
@@ -588,6 +688,8 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
     bnd::PathBuf P = path_buf(c);
     bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
     const int gt = grid_trace(c), gs = c->n_cu * 8;
+    c->stats.ensure(8 * sizeof(unsigned long long));
+    const bnd::Buffers BF = buffers(c);
     KernelTimer& T = c->timer;
     bnd::MltParamsDev dp{p.width, p.height, p.frame_id, p.n_bootstrap, p.large_step_prob, 0.0f, 0.0f};
     const uint32_t bin_cap = (uint32_t)c->path_cap;
@@ -602,9 +704,8 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
             const int nxt = 1 - cur;
             { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
             { Timed tm(T, "trace_closest", s);
-              bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p,
-                                        (uint32_t*)c->spill.p, c->spill_stride, (uint32_t)c->refill,
-                                        (unsigned long long*)c->stats.p, gt, s); }
+              bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p, BF,
+                                        (uint32_t)c->refill, gt, s); }
             { Timed tm(T, "bin", s);
               bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
             { Timed tm(T, "shade_mlt", s);
@@ -612,13 +713,11 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
                                     (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p,
                                     depth, p.max_depth, p.rr_depth, gs, s); }
             { Timed tm(T, "trace_shadow", s);
-              bnd::launch_trace_shadow(false, sc.dev, P, ctl, (const uint32_t*)c->shq.p, (uint32_t*)c->spill.p,
-                                       c->spill_stride, (uint32_t)c->refill, (unsigned long long*)c->stats.p, gt,
-                                       s); }
+              bnd::launch_trace_shadow(false, sc.dev, P, ctl, (const uint32_t*)c->shq.p, BF, (uint32_t)c->refill,
+                                       gt, s); }
             cur = nxt;
         }
     };
-    c->stats.ensure(8 * sizeof(unsigned long long));
 
     // PSSMLT.fs:382-392: bootstrap
     for (int id0 = 0; id0 < p.n_bootstrap; id0 += (int)cap) {
@@ -688,7 +787,8 @@ int bn_struct_sizes(int32_t* out, int32_t n) {
                          (int32_t)sizeof(bn_mlt_params), (int32_t)sizeof(bn_mlt_stats),
                          (int32_t)sizeof(bn_trace_stats), (int32_t)sizeof(bn_bvh_node),
                          (int32_t)sizeof(bn_instance_upload), (int32_t)sizeof(bn_batch_item),
-                         (int32_t)sizeof(bn_batch_stats), (int32_t)sizeof(bn_kernel_time)};
+                         (int32_t)sizeof(bn_batch_stats), (int32_t)sizeof(bn_kernel_time),
+                         (int32_t)sizeof(bn_tune_result)};
     const int m = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
     return m;
@@ -955,6 +1055,12 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
             c->mlt_chunk = (size_t)value;
         } else if (k == "graph") {
             c->use_graph = value != 0;
+        } else if (k == "stack_k") {
+            if (value != 16 && value != 32) throw Error(BN_E_INVALID, "stack_k must be 16 or 32");
+            c->stack_k = (int)value;
+        } else if (k == "batch_pack") {
+            if (value < 0 || value > 1) throw Error(BN_E_INVALID, "batch_pack must be 0 or 1");
+            c->batch_pack = (int)value;
         } else if (k == "refill") {
             if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
             c->refill = (int)value;
@@ -1022,5 +1128,14 @@ extern "C" int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t 
         if (!c) throw Error(BN_E_INVALID, "null context");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         render_batch(c, items, n, films, stats);
+    });
+}
+
+extern "C" int bn_autotune(bn_ctx* ctx, const bn_batch_item* items, int32_t n, int32_t flags, bn_tune_result* out) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c) throw Error(BN_E_INVALID, "null context");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        autotune(c, items, n, flags, out);
     });
 }

@@ -75,7 +75,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int trace_blocks_per_cu = 4;
     int refill = 48;                      // idle lanes that trigger a refill
-    size_t chunk_slots = (size_t)8 << 20;
+    size_t chunk_slots = (size_t)64 << 20;
     std::vector<BlasRec> blas;
     std::vector<std::unique_ptr<SceneRec>> scenes;
     // path state
@@ -87,6 +87,8 @@ struct Ctx {
     KernelTimer timer;
     // batched rounds
     bool use_graph = true;
+    int stack_k = 32;                      // LDS traversal stack window
+    int batch_pack = 0;                    // 0 item-major, 1 fair share
     DevBuf b_scenes, b_items, b_hdr;
     std::map<std::string, GraphExec> graphs;   // captured bounce loops by configuration
     // PSSMLT chain state

@@ -8,7 +8,8 @@
 namespace bnd {
 
 constexpr int kBins = 5;          // material kinds 0..3 + light-only instances
-constexpr int kStackK = 32;       // LDS window of the per-lane traversal stack
+constexpr int kStackK = 32;       // default LDS window of the per-lane traversal stack
+constexpr int kStackKMin = 16;    // smallest window (sizes the spill area)
 constexpr int kStackMax = 144;    // TLAS + BLAS depth <= 2 * (64 + 1) entries
 constexpr int kBlock = 256;
 
@@ -78,6 +79,7 @@ struct Buffers {
     uint32_t bin_cap;
     uint32_t* spill;
     size_t spill_stride;
+    int stack_k;              // LDS stack window: 16 or 32 entries
     unsigned long long* stats;
     float* film;
 };
@@ -101,11 +103,9 @@ void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint
 void launch_reset(Ctl* ctl, int nxt, hipStream_t st);
 void launch_ctl_clear(Ctl* ctl, hipStream_t st);
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
-                          hipStream_t st);
+                          const Buffers& B, uint32_t refill, int grid, hipStream_t st);
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
-                         hipStream_t st);
+                         const Buffers& B, uint32_t refill, int grid, hipStream_t st);
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st);
 

@@ -149,13 +149,13 @@ __global__ void k_reset(Ctl* ctl, int nxt) {
 // Lane-refilling persistent traversal: a wave keeps up to 64 rays in flight,
 // advances each by one item per iteration and refills idle lanes from the
 // queue once at least `refill` of them are idle (one atomic per wave).
-template <bool ANY, bool COUNT, typename Init, typename Done>
+template <bool ANY, bool COUNT, int K, typename Init, typename Done>
 __device__ __forceinline__ void persistent_trace(uint32_t* fetch_ctr, uint32_t count, uint32_t* stk, uint32_t* spill,
                                                  size_t spill_stride, uint32_t refill, TraceCount& tc, Init init,
                                                  Done done) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Traversal<kStackK> tr;
-    tr.st = LaneStack<kStackK>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    Traversal<K> tr;
+    tr.st = LaneStack<K>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
     bool active = false, more = true;
     uint32_t item = 0;
     for (;;) {
@@ -189,20 +189,20 @@ __device__ __forceinline__ void persistent_trace(uint32_t* fetch_ctr, uint32_t c
 // BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  The
 // material bin of the hit (kind of the instance's material; light-only
 // instances in the last bin, -1 for a miss) travels in the hit record.
-template <bool COUNT, bool BATCH>
+template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* spill, size_t spill_stride,
                                                        uint32_t refill, unsigned long long* stats) {
-    __shared__ uint32_t stk[kStackK * 256];
+    __shared__ uint32_t stk[K * 256];
     const uint32_t count = ctl->q_count[cur];
     TraceCount tc{0, 0, 0};
-    persistent_trace<false, COUNT>(
+    persistent_trace<false, COUNT, K>(
         &ctl->fetch[0], count, stk, spill, spill_stride, refill, tc,
-        [&](Traversal<kStackK>& tr, uint32_t i) {
+        [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = queue[i];
             tr.start(W.scene(W.item_of(P, slot)), {xyz(P.o[slot]), xyz(P.d[slot])}, kInf);
         },
-        [&](Traversal<kStackK>& tr, uint32_t i, bool fin, bool) {
+        [&](Traversal<K>& tr, uint32_t i, bool fin, bool) {
             if (fin) {
                 const uint32_t slot = queue[i];
                 int bin = -1;
@@ -353,23 +353,23 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 // ----------------------------------------------------------- trace shadow
 // aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
 // an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
-template <bool COUNT, bool BATCH>
+template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats) {
-    __shared__ uint32_t stk[kStackK * 256];
+    __shared__ uint32_t stk[K * 256];
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
-    persistent_trace<true, COUNT>(
+    persistent_trace<true, COUNT, K>(
         &ctl->fetch[1], count, stk, spill, spill_stride, refill, tc,
-        [&](Traversal<kStackK>& tr, uint32_t i) {
+        [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
             tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
             // a NaN t max fails every slab test of the reference: empty traversal
             if (!(o4.w == o4.w)) tr.st.sp = 0;
         },
-        [&](Traversal<kStackK>& tr, uint32_t i, bool fin, bool found) {
+        [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
             if (fin && !found) {
                 const uint32_t slot = shq[i];
                 const V3 a = xyz(P.nee_a[slot]), b = xyz(P.nee_b[slot]);
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
     __shared__ uint32_t stk[kStackK * 256];
     const uint32_t n = (uint32_t)width * (uint32_t)height;
     TraceCount tc{0, 0, 0};
-    persistent_trace<false, false>(
+    persistent_trace<false, false, kStackK>(
         fetch, n, stk, spill, spill_stride, 16u, tc,
         [&](Traversal<kStackK>& tr, uint32_t p) {
             const int x = (int)(p % (uint32_t)width), y = (int)(p / (uint32_t)width);
@@ -478,6 +478,28 @@ static Work<true> batch_work(const Launch& L) {
     return w;
 }
 
+// Trace launches dispatched over the LDS stack window (16 or 32 entries).
+template <bool BATCH>
+static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, int cur,
+                             const uint32_t* queue, const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
+#define BN_TC(C, KK)                                                                                            \
+    hipLaunchKernelGGL((k_trace_closest<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, \
+                       B.spill, B.spill_stride, refill, B.stats)
+    if (k == 16) { if (count) BN_TC(true, 16); else BN_TC(false, 16); }
+    else { if (count) BN_TC(true, 32); else BN_TC(false, 32); }
+#undef BN_TC
+}
+template <bool BATCH>
+static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
+                            const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
+#define BN_TS(C, KK)                                                                                              \
+    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, B.spill, \
+                       B.spill_stride, refill, B.stats)
+    if (k == 16) { if (count) BN_TS(true, 16); else BN_TS(false, 16); }
+    else { if (count) BN_TS(true, 32); else BN_TS(false, 32); }
+#undef BN_TS
+}
+
 template <bool BATCH>
 static void enqueue_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
                       int depth_n, bool count, LaunchHook* hook) {
@@ -495,26 +517,16 @@ static void enqueue_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, C
     for (int depth = 0; depth < depth_n; ++depth) {
         const int nxt = 1 - cur;
         { Bracket b(hook, "reset", st); hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt); }
-        { Bracket b(hook, "trace_closest", st);
-          if (count)
-              hipLaunchKernelGGL((k_trace_closest<true, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl, cur,
-                                 B.queue[cur], B.spill, B.spill_stride, L.refill, B.stats);
-          else
-              hipLaunchKernelGGL((k_trace_closest<false, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
-                                 cur, B.queue[cur], B.spill, B.spill_stride, L.refill, B.stats); }
+        { Bracket b(hook, "trace_closest", st); launch_closest_t<BATCH>(count, B.stack_k, W, P, ctl, cur, B.queue[cur],
+                                                                         B, L.refill, L.grid_trace, st); }
         { Bracket b(hook, "bin", st);
           hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, cur, B.queue[cur], B.bins,
                              B.bin_cap); }
         { Bracket b(hook, "shade", st);
           hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
                              B.queue[nxt], nxt, B.shq, depth); }
-        { Bracket b(hook, "trace_shadow", st);
-          if (count)
-              hipLaunchKernelGGL((k_trace_shadow<true, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
-                                 B.shq, B.spill, B.spill_stride, L.refill, B.stats);
-          else
-              hipLaunchKernelGGL((k_trace_shadow<false, BATCH>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
-                                 B.shq, B.spill, B.spill_stride, L.refill, B.stats); }
+        { Bracket b(hook, "trace_shadow", st); launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, B, L.refill,
+                                                                       L.grid_trace, st); }
         if (hook) hook->after_bounce(cur);
         cur = nxt;
     }
@@ -535,8 +547,7 @@ void enqueue_normal(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& 
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
     hipLaunchKernelGGL(k_raygen<false>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.queue[0]);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, 1);
-    hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl, 0,
-                       B.queue[0], B.spill, B.spill_stride, L.refill, B.stats);
+    launch_closest_t<false>(false, B.stack_k, W, P, ctl, 0, B.queue[0], B, L.refill, L.grid_trace, st);
     hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, 0, B.queue[0], B.bins, B.bin_cap);
     hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap);
     hipLaunchKernelGGL(k_film<false>, dim3((L.n_pix + 255) / 256), dim3(256), 0, st, W, P, B.film);
@@ -554,30 +565,18 @@ void launch_ctl_clear(Ctl* ctl, hipStream_t st) {
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
 }
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
-                          uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
-                          hipStream_t st) {
+                          const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
     Work<false> W;
     W.S = S;
     W.it = RoundItem{};
-    if (count)
-        hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, spill,
-                           spill_stride, refill, stats);
-    else
-        hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, spill,
-                           spill_stride, refill, stats);
+    launch_closest_t<false>(count, B.stack_k, W, P, ctl, cur, queue, B, refill, grid, st);
 }
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         uint32_t* spill, size_t spill_stride, uint32_t refill, unsigned long long* stats, int grid,
-                         hipStream_t st) {
+                         const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
     Work<false> W;
     W.S = S;
     W.it = RoundItem{};
-    if (count)
-        hipLaunchKernelGGL((k_trace_shadow<true, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, spill,
-                           spill_stride, refill, stats);
-    else
-        hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, spill,
-                           spill_stride, refill, stats);
+    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, B, refill, grid, st);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {

@@ -57,6 +57,8 @@ _SIGS = {
                          C.c_int),
     "bn_render_batch": ([C.c_void_p, C.POINTER(abi.BatchItem), C.c_int32, C.POINTER(f32p),
                          C.POINTER(abi.BatchStats)], C.c_int),
+    "bn_autotune": ([C.c_void_p, C.POINTER(abi.BatchItem), C.c_int32, C.c_int32, C.POINTER(abi.TuneResult)],
+                    C.c_int),
     "bn_pssmlt_last": ([C.c_void_p, f32p, C.c_int32, i32p, C.c_int32], C.c_int),
 }
 
@@ -148,6 +150,18 @@ class Context:
         st = abi.BatchStats()
         check(self.lib.bn_render_batch(self.h, items, n, ptrs, C.byref(st)))
         return films, {"rounds": st.rounds, "graph": st.graph}
+
+
+    def autotune(self, jobs, batched: bool = False):
+        """Tune launch parameters on a workload of (DeviceScene, PTParams);
+        the chosen setting stays on the context.  Returns it as a dict."""
+        n = len(jobs)
+        items = (abi.BatchItem * n)()
+        for k, (ds, p) in enumerate(jobs):
+            items[k] = abi.BatchItem(ds.id, p)
+        r = abi.TuneResult()
+        check(self.lib.bn_autotune(self.h, items, n, abi.BN_TUNE_BATCH if batched else 0, C.byref(r)))
+        return {n_: getattr(r, n_) for n_, _ in abi.TuneResult._fields_}
 
 
 class DeviceScene:

@@ -289,11 +289,13 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
 /* Tunables and instrumentation (no reference counterpart).  Keys:
  *   "timing"               1: record a HIP event pair around every kernel
  *                          launch of the render calls, on the context's stream
- *   "chunk_slots"          path slots in flight per bounce loop (default 8M)
+ *   "chunk_slots"          path slots in flight per bounce loop (default 64M)
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
  *   "refill"               idle lanes of a wave that trigger a refill (1..64)
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
- *   "graph"                1 (default): replay batched rounds as a HIP graph */
+ *   "stack_k"              LDS window of the traversal stack: 16 or 32 (default)
+ *   "graph"                1 (default): replay batched rounds as a HIP graph
+ *   "batch_pack"           0 (default): pack rounds item-major; 1: fair share */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {
@@ -305,6 +307,19 @@ typedef struct bn_kernel_time {
  * count out). */
 int bn_kernel_times(bn_ctx* ctx, bn_kernel_time* out, int32_t* n);
 int bn_reset_kernel_times(bn_ctx* ctx);
+
+/* Launch-parameter autotuner (no reference counterpart).  Times the given
+ * workload -- plain renders of every item in turn, or with BN_TUNE_BATCH one
+ * batched render -- over trace_blocks_per_cu, refill, stack_k and
+ * chunk_slots by coordinate descent, and leaves the fastest setting on the
+ * context.  Images do not depend on these parameters. */
+#define BN_TUNE_BATCH 1
+typedef struct bn_tune_result {
+    int32_t trace_blocks_per_cu, refill, stack_k, trials;
+    int64_t chunk_slots;
+    float baseline_ms, best_ms;   /* workload time at the entry / chosen setting */
+} bn_tune_result;
+int bn_autotune(bn_ctx* ctx, const bn_batch_item* items, int32_t n, int32_t flags, bn_tune_result* out);
 
 /* PSSMLTIntegrator.Render (Extensions/Integrator/PSSMLT.fs:379-414).  The film
  * is cleared first (Film.Clear, Render.fs:11) and accumulated with device

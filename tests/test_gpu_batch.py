@@ -46,7 +46,7 @@ def test_batch_many_rounds_equals_plain(ctx, jobs):
     try:
         films, st = ctx.render_batch([(ds, p) for ds, p, _ in jobs])
     finally:
-        ctx.set_option("chunk_slots", 8 << 20)
+        ctx.set_option("chunk_slots", 64 << 20)
     assert st["rounds"] > 3
     for (ds, p, _), f in zip(jobs, films):
         assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
