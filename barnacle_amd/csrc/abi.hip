@@ -283,9 +283,12 @@ void ensure_paths(Ctx* c, size_t cap) {
     f4(c->p_o); f4(c->p_d); f4(c->p_beta); f4(c->p_L); f4(c->p_sh_o); f4(c->p_sh_d); f4(c->p_nee_a); f4(c->p_nee_b);
     c->p_rng.ensure(cap * 4);
     c->p_item.ensure(cap * 4);
+    c->p_pid.ensure(cap * 4);
     c->p_hit.ensure(cap * sizeof(int4));
     c->queue[0].ensure(cap * 4);
     c->queue[1].ensure(cap * 4);
+    c->freel[0].ensure(cap * 4);
+    c->freel[1].ensure(cap * 4);
     c->shq.ensure(cap * 4);
     c->bins.ensure(cap * 4 * bnd::kBins);
     c->path_cap = cap;
@@ -299,6 +302,7 @@ bnd::PathBuf path_buf(Ctx* c) {
     P.sh_o = (float4*)c->p_sh_o.p; P.sh_d = (float4*)c->p_sh_d.p;
     P.nee_a = (float4*)c->p_nee_a.p; P.nee_b = (float4*)c->p_nee_b.p;
     P.item = (int*)c->p_item.p;
+    P.pid = (uint32_t*)c->p_pid.p;
     return P;
 }
 
@@ -314,6 +318,8 @@ bnd::Buffers buffers(Ctx* c) {
     bnd::Buffers B;
     B.queue[0] = (uint32_t*)c->queue[0].p;
     B.queue[1] = (uint32_t*)c->queue[1].p;
+    B.freel[0] = (uint32_t*)c->freel[0].p;
+    B.freel[1] = (uint32_t*)c->freel[1].p;
     B.shq = (uint32_t*)c->shq.p;
     B.bins = (uint32_t*)c->bins.p;
     B.bin_cap = (uint32_t)c->path_cap;
@@ -322,6 +328,9 @@ bnd::Buffers buffers(Ctx* c) {
     B.stack_k = c->stack_k;
     B.stats = (unsigned long long*)c->stats.p;
     B.film = (float*)c->film.p;
+    B.L_out = (float4*)c->L_out.p;
+    c->fetch_seg.ensure((size_t)2 * bnd::kFetchSegs * bnd::kSegStride * 4);
+    B.fetch_seg = (uint32_t*)c->fetch_seg.p;
     return B;
 }
 
@@ -362,18 +371,59 @@ void check_pt_params(const bn_pt_params& p) {
     if ((int64_t)p.width * p.height > ((int64_t)1 << 28)) throw Error(BN_E_CAPACITY, "film larger than 2^28 pixels");
 }
 
-// One frame of PathTracingIntegrator (or the Normal view), in chunks of
-// whole samples of all pixels, chunk after chunk in sample order.
+constexpr int kIterBlock = 8;     // iterations between two drain checks (even)
+
+// Paths per round.  By default a round holds no more paths than there are
+// slots, so every path of a round is in flight from the first iteration:
+// reusing slots within a round scatters new paths over the slot arrays and
+// costs more in uncoalesced path-state access than it saves in bounce tails.
+size_t window_paths(const Ctx* c) { return c->window_paths ? c->window_paths : c->chunk_slots; }
+
+// One round: its paths stream through L.n_slots slots until every path has
+// retired, checked on the host after every block of kIterBlock iterations
+// (a captured graph of one block when `block` is set).
+void run_round(Ctx* c, bool batch, bnd::Integrator kind, const bnd::Launch& L, const bnd::PathBuf& P, bnd::Ctl* ctl,
+               const bnd::Buffers& B, uint32_t n_paths, int max_depth, bool count, Hook* hook,
+               hipGraphExec_t block) {
+    hipStream_t s = L.stream;
+    bnd::enqueue_round_begin(batch, L, P, ctl, B, hook);
+    // every path needs at most max_depth + 1 iterations once it has a slot
+    const int64_t limit = ((int64_t)n_paths / L.n_slots + 2) * (int64_t)(max_depth + 2) * 2 + 4 * kIterBlock;
+    int64_t iters = 0;
+    for (;;) {
+        if (block) {
+            c->timer.begin("iteration_block", s);
+            hip_check(hipGraphLaunch(block, s), "graph launch");
+            c->timer.end(s);
+        } else {
+            bnd::enqueue_iterations(batch, kind, L, P, ctl, B, kIterBlock, count, hook);
+        }
+        iters += kIterBlock;
+        bnd::Ctl h;
+        hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "ctl readback");
+        hip_check(hipStreamSynchronize(s), "iteration block");
+        if (h.next_pid >= n_paths && h.q_count[0] == 0) break;
+        if (iters > limit) throw Error(BN_E_DEVICE, "round did not drain");
+    }
+    bnd::enqueue_round_end(batch, L, P, ctl, B, hook);
+}
+
+// One frame of PathTracingIntegrator (or the Normal view), in rounds of
+// whole samples of all pixels, round after round in sample order.
 void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, float* film_host, bn_trace_stats* stats) {
     check_pt_params(p);
     if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
     hipStream_t s = c->stream;
     camera_constants(sc, p.width, p.height);
     bnd::RoundItem it = plain_item(p);
-    const size_t want = std::max<size_t>((size_t)it.p_pad, c->chunk_slots);
+    if (normal) it.max_depth = 1;                          // one bounce, whatever max-depth says
+    const size_t want = std::max<size_t>((size_t)it.p_pad, window_paths(c));
     const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / it.p_pad));
-    ensure_paths(c, (size_t)ns * it.p_pad);
+    const size_t round_paths = (size_t)ns * it.p_pad;
+    const size_t slots = std::min(round_paths, c->chunk_slots);
+    ensure_paths(c, slots);
     ensure_spill(c);
+    c->L_out.ensure(round_paths * sizeof(float4));
     const size_t film_n = (size_t)p.width * p.height * 3;
     c->film.ensure(film_n * 4);
     c->ctl.ensure(sizeof(bnd::Ctl));
@@ -391,14 +441,15 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
     L.refill = (uint32_t)c->refill;
     L.stream = s;
     L.n_pix = (uint32_t)(p.width * p.height);
+    L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
     for (int s0 = 0; s0 < p.spp; s0 += ns) {
         it.s0 = s0;
         it.ns = std::min(ns, p.spp - s0);
-        it.slot_count = it.ns * it.p_pad;
+        it.path_count = it.ns * it.p_pad;
         L.item = it;
-        if (normal) bnd::enqueue_normal(L, P, ctl, B);
-        else bnd::enqueue_bounce_loop(false, L, P, ctl, B, p.max_depth, stats != nullptr, &hook);
+        run_round(c, false, normal ? bnd::Integrator::Normal : bnd::Integrator::Path, L, P, ctl, B,
+                  (uint32_t)it.path_count, it.max_depth, stats != nullptr, &hook, nullptr);
     }
     hip_check(hipGetLastError(), "kernel launch");
     if (film_host) hip_check(hipMemcpyAsync(film_host, c->film.p, film_n * 4, hipMemcpyDeviceToHost, s), "film D2H");
@@ -410,22 +461,26 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
         acc.nodes_closest = (int64_t)h[0]; acc.tris_closest = (int64_t)h[1]; acc.insts_closest = (int64_t)h[2];
         acc.nodes_shadow = (int64_t)h[3]; acc.tris_shadow = (int64_t)h[4]; acc.insts_shadow = (int64_t)h[5];
         acc.paths = (int64_t)p.width * p.height * p.spp;
+        // placeholder paths (padding pixels; every path at max-depth 0) are
+        // queued for one iteration but trace nothing
+        const int64_t pix = (int64_t)p.width * p.height;
+        acc.rays_closest -= (int64_t)p.spp * (it.p_pad - pix) + (p.max_depth <= 0 ? (int64_t)p.spp * pix : 0);
         *stats = acc;
     }
 }
 
 // Several renders in one launch sequence per round (see kernels_pt.hip).
-// Rounds pack whole samples of the items into the slot capacity, every item
-// in sample order; a round's bounce loop is captured once as a graph and
-// replayed with the next round table.
+// Rounds pack whole samples of the items, every item in sample order, up to
+// window_paths paths; all paths of a round share the slot pool, so the items
+// fill each other's bounce tails.  A block of iterations is captured once as
+// a graph and replayed for every block of every round.
 void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films, bn_batch_stats* stats) {
     if (!items || n <= 0) throw Error(BN_E_INVALID, "empty batch");
     if (n > 4096) throw Error(BN_E_CAPACITY, "more than 4096 batch items");
     hipStream_t s = c->stream;
     std::vector<bnd::RoundItem> base(n);
     std::vector<bnd::DevScene> scenes(n);
-    size_t film_total = 0, slots_total = 0;
-    int depth_n = 0;
+    size_t film_total = 0, paths_total = 0;
     size_t max_pad = 0;
     for (int k = 0; k < n; ++k) {
         const bn_pt_params& p = items[k].params;
@@ -439,13 +494,15 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         base[k].film_offset = (int)film_total;
         film_total += (size_t)p.width * p.height * 3;
         if (film_total > ((size_t)1 << 31)) throw Error(BN_E_CAPACITY, "batch films exceed 2^31 floats");
-        depth_n = std::max(depth_n, p.max_depth);
         max_pad = std::max(max_pad, (size_t)base[k].p_pad);
-        slots_total += (size_t)base[k].p_pad * (size_t)std::max(p.spp, 0);
+        paths_total += (size_t)base[k].p_pad * (size_t)std::max(p.spp, 0);
     }
-    const size_t cap = std::max(max_pad, std::min(c->chunk_slots, slots_total));
-    ensure_paths(c, cap);
+    const size_t cap = std::max(max_pad, std::min(window_paths(c), paths_total));
+    if (cap > ((size_t)1 << 31)) throw Error(BN_E_CAPACITY, "round larger than 2^31 paths");
+    const size_t slots = std::min(cap, c->chunk_slots);
+    ensure_paths(c, slots);
     ensure_spill(c);
+    c->L_out.ensure(cap * sizeof(float4));
     c->film.ensure(film_total * 4);
     c->ctl.ensure(sizeof(bnd::Ctl));
     c->stats.ensure(8 * sizeof(unsigned long long));
@@ -463,17 +520,19 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.stream = s;
+    L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
     hipGraphExec_t exec = nullptr;
     if (c->use_graph) {
-        const std::string key = "batch:" + std::to_string(depth_n) + ":" + std::to_string(L.grid_trace) + ":" +
-                                std::to_string(L.refill) + ":" + std::to_string(B.stack_k) + ":" + std::to_string((uintptr_t)c->b_items.p) + ":" +
-                                std::to_string((uintptr_t)c->b_scenes.p) + ":" + std::to_string((uintptr_t)c->film.p);
+        const std::string key = "block:" + std::to_string(L.grid_trace) + ":" + std::to_string(L.refill) + ":" +
+                                std::to_string(B.stack_k) + ":" + std::to_string(L.n_slots) + ":" +
+                                std::to_string((uintptr_t)c->b_items.p) + ":" +
+                                std::to_string((uintptr_t)c->b_scenes.p) + ":" + std::to_string((uintptr_t)c->L_out.p);
         auto f = c->graphs.find(key);
         if (f == c->graphs.end()) {
             hipGraph_t g;
             hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-            bnd::enqueue_bounce_loop(true, L, P, ctl, B, depth_n, false, nullptr);
+            bnd::enqueue_iterations(true, bnd::Integrator::Path, L, P, ctl, B, kIterBlock, false, nullptr);
             hip_check(hipStreamEndCapture(s, &g), "end capture");
             hipGraphExec_t e;
             hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "graph instantiate");
@@ -482,6 +541,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         }
         exec = f->second.e;
     }
+    Hook hook(c, ctl, nullptr);
     std::vector<int> next(n, 0);
     std::vector<bnd::RoundItem> round;
     int rounds = 0;
@@ -518,34 +578,28 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
                     used += (size_t)t * pad;
                 }
         }
-        int slot = 0, pix = 0;
+        int path = 0, pix = 0, round_depth = 0;
         for (int k = 0; k < n; ++k) {
             if (take[k] == 0) continue;
             bnd::RoundItem it = base[k];
             it.s0 = next[k];
             it.ns = take[k];
-            it.slot_base = slot;
-            it.slot_count = take[k] * it.p_pad;
+            it.path_base = path;
+            it.path_count = take[k] * it.p_pad;
             pix += it.width * it.height;
             it.pix_end = pix;
-            slot += it.slot_count;
+            path += it.path_count;
             next[k] += take[k];
+            round_depth = std::max(round_depth, it.max_depth);
             round.push_back(it);
         }
-        bnd::RoundHeader h{(int)round.size(), (uint32_t)slot, (uint32_t)pix, 0};
-        // the stream orders these copies before the round's kernels
+        bnd::RoundHeader h{(int)round.size(), (uint32_t)path, (uint32_t)pix, 0};
+        // the stream orders these copies before the round's kernels; run_round
+        // synchronises before the host tables are reused
         hip_check(hipMemcpyAsync(c->b_items.p, round.data(), round.size() * sizeof(bnd::RoundItem),
                                  hipMemcpyHostToDevice, s), "round table");
         hip_check(hipMemcpyAsync(c->b_hdr.p, &h, sizeof(h), hipMemcpyHostToDevice, s), "round header");
-        if (exec) {
-            c->timer.begin("batch_round", s);
-            hip_check(hipGraphLaunch(exec, s), "graph launch");
-            c->timer.end(s);
-        } else {
-            Hook hook(c, ctl, nullptr);
-            bnd::enqueue_bounce_loop(true, L, P, ctl, B, depth_n, false, &hook);
-        }
-        // the host round table is reused next round: wait for the copies
+        run_round(c, true, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)path, round_depth, false, &hook, exec);
         hip_check(hipStreamSynchronize(s), "round");
         ++rounds;
     }
@@ -581,7 +635,7 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     // path state + queues + bins per slot
-    const size_t per_slot = 8 * sizeof(float4) + sizeof(int4) + 4 * 5 + 4 * bnd::kBins;
+    const size_t per_slot = 8 * sizeof(float4) + sizeof(int4) + 4 * 8 + 4 * bnd::kBins;
     const size_t slot_limit = (free_b + c->path_cap * per_slot) / 4 / per_slot;
 
     auto run = [&]() -> double {
@@ -702,12 +756,13 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
         int cur = 0;
         for (int depth = 0; depth < p.max_depth; ++depth) {
             const int nxt = 1 - cur;
-            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, s); }
+            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, BF.fetch_seg, s); }
             { Timed tm(T, "trace_closest", s);
               bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p, BF,
                                         (uint32_t)c->refill, gt, s); }
             { Timed tm(T, "bin", s);
-              bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, gs, s); }
+              bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, nullptr,
+                              gs, s); }
             { Timed tm(T, "shade_mlt", s);
               bnd::launch_shade_mlt(fresh, sc.dev, P, M, ctl, (const uint32_t*)c->bins.p, bin_cap,
                                     (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p,
@@ -1055,6 +1110,10 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
             c->mlt_chunk = (size_t)value;
         } else if (k == "graph") {
             c->use_graph = value != 0;
+        } else if (k == "window_paths") {
+            if (value != 0 && (value < 64 || value > ((int64_t)1 << 31)))
+                throw Error(BN_E_INVALID, "window_paths out of range");
+            c->window_paths = (size_t)value;
         } else if (k == "stack_k") {
             if (value != 16 && value != 32) throw Error(BN_E_INVALID, "stack_k must be 16 or 32");
             c->stack_k = (int)value;

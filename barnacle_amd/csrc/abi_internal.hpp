@@ -75,13 +75,14 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int trace_blocks_per_cu = 4;
     int refill = 48;                      // idle lanes that trigger a refill
-    size_t chunk_slots = (size_t)64 << 20;
+    size_t chunk_slots = (size_t)64 << 20;       // path slots in flight
+    size_t window_paths = 0;                     // paths per round; 0: as many as slots
     std::vector<BlasRec> blas;
     std::vector<std::unique_ptr<SceneRec>> scenes;
     // path state
     size_t path_cap = 0;
-    DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b, p_item;
-    DevBuf queue[2], shq, bins, spill, film, ctl, stats;
+    DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b, p_item, p_pid;
+    DevBuf queue[2], freel[2], shq, bins, spill, film, ctl, stats, L_out, fetch_seg;
     size_t spill_stride = 0;
     DevBuf out_a, out_b, out_c;
     KernelTimer timer;

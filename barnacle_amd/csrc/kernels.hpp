@@ -12,10 +12,14 @@ constexpr int kStackK = 32;       // default LDS window of the per-lane traversa
 constexpr int kStackKMin = 16;    // smallest window (sizes the spill area)
 constexpr int kStackMax = 144;    // TLAS + BLAS depth <= 2 * (64 + 1) entries
 constexpr int kBlock = 256;
+constexpr int kFetchSegs = 64;    // fetch counters per trace queue
+constexpr int kSegStride = 64;    // words between two fetch counters (own cache lines)
+
+constexpr uint32_t kNoPath = 0xffffffffu;
 
 // Path state, struct of arrays indexed by slot.
 struct PathBuf {
-    float4* o;        // ray origin
+    float4* o;        // ray origin, w = bounce depth of the path (int bits)
     float4* d;        // ray direction, w = pdf of the last BSDF sample
     float4* beta;
     float4* L;
@@ -25,22 +29,25 @@ struct PathBuf {
     float4* sh_d;
     float4* nee_a;    // beta * f
     float4* nee_b;    // Le / (pdf_b + pdf_l)
-    int* item;        // render item of the slot (batched rounds)
+    int* item;        // render item of the slot's path (batched rounds)
+    uint32_t* pid;    // path id of the slot within its round, kNoPath if idle
 };
 
-// Device-side counters of one bounce loop.
+// Device-side counters of the iterations of a round.
 struct Ctl {
-    uint32_t q_count[2];
+    uint32_t q_count[2];      // active queues (this iteration, next iteration)
     uint32_t bin_count[kBins];
     uint32_t sh_count;
     uint32_t fetch[4];
-    uint32_t pad[4];
+    uint32_t free_count[2];   // free-slot lists (read by this iteration's regen, filled for the next)
+    uint32_t next_pid;        // next path id of the round to issue (may run past the end)
+    uint32_t pad;
 };
 
 // One render item of a round: samples [s0, s0 + ns) of every pixel of one
-// film, in slots [slot_base, slot_base + ns * p_pad).
+// film, path ids [path_base, path_base + ns * p_pad) of the round.
 struct RoundItem {
-    int slot_base, slot_count;
+    int path_base, path_count;
     int scene;                // index into the scene table (batch)
     int width, height, tiles_x, p_pad;
     int s0, ns, spp, frame_id;
@@ -51,15 +58,15 @@ struct RoundItem {
 };
 
 // Sizes of a batched round, read by the kernels from device memory so that a
-// captured bounce loop can be replayed for every round.
+// captured block of iterations can be replayed for every round.
 struct RoundHeader {
     int n_items;
-    uint32_t n_slots;
+    uint32_t n_paths;
     uint32_t n_pix;
     int pad;
 };
 
-// Everything a bounce loop needs besides buffers.
+// Everything a round needs besides buffers.
 struct Launch {
     DevScene scene;           // plain mode
     RoundItem item;           // plain mode
@@ -67,6 +74,7 @@ struct Launch {
     const RoundItem* items;   // batch mode (device)
     const DevScene* scenes;   // batch mode (device)
     uint32_t n_pix;           // plain mode
+    uint32_t n_slots;         // path slots in flight
     int grid_trace, grid_shade;
     uint32_t refill;
     hipStream_t stream;
@@ -74,6 +82,7 @@ struct Launch {
 
 struct Buffers {
     uint32_t* queue[2];
+    uint32_t* freel[2];
     uint32_t* shq;
     uint32_t* bins;
     uint32_t bin_cap;
@@ -82,9 +91,11 @@ struct Buffers {
     int stack_k;              // LDS stack window: 16 or 32 entries
     unsigned long long* stats;
     float* film;
+    float4* L_out;            // radiance per path id of the round
+    uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters (closest, shadow)
 };
 
-// Called around every launch of a bounce loop (timing, counters).
+// Called around every launch of a round (timing, counters).
 struct LaunchHook {
     virtual void begin(const char* name, hipStream_t st) = 0;
     virtual void end(hipStream_t st) = 0;
@@ -92,15 +103,22 @@ struct LaunchHook {
     virtual ~LaunchHook() = default;
 };
 
-// ctl_clear, raygen, [reset, trace_closest, bin, shade, trace_shadow] x depth_n, film
-void enqueue_bounce_loop(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, int depth_n,
-                         bool count, LaunchHook* hook);
-// NormalIntegrator frame: raygen, closest hit, normal shading, film.
-void enqueue_normal(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B);
+// A round streams its paths through the slots: every iteration refills the
+// free slots with new paths (regen), then traces, bins, shades and traces
+// shadow rays for all active paths, whatever their depth.
+//   begin:      counter clear, every slot free
+//   iterations: n (even) x [regen, prep, trace_closest, bin, shade, trace_shadow]
+//   end:        retire the last paths, fold the round into the films
+enum class Integrator { Path, Normal };
+void enqueue_round_begin(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                         LaunchHook* hook);
+void enqueue_iterations(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                        int n, bool count, LaunchHook* hook);
+void enqueue_round_end(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, LaunchHook* hook);
 
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                int grid, hipStream_t st);
-void launch_reset(Ctl* ctl, int nxt, hipStream_t st);
+                uint32_t* freel, int grid, hipStream_t st);
+void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st);
 void launch_ctl_clear(Ctl* ctl, hipStream_t st);
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
                           const Buffers& B, uint32_t refill, int grid, hipStream_t st);

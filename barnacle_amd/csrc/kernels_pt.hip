@@ -1,20 +1,25 @@
 // kernels_pt.hip -- wavefront path tracing on gfx950.
 //
 // One pass of PathTracingIntegrator.Li (Extensions/Integrator/PathTracing.fs:14-81)
-// is split into per-bounce kernels over compacted queues of path slots:
+// is split into per-bounce kernels over compacted queues of path slots.  A
+// round streams its paths through a fixed pool of slots; every iteration
 //
-//   raygen -> [ reset -> trace_closest -> bin -> shade -> trace_shadow ] x depth -> film
+//   regen -> prep -> trace_closest -> bin -> shade -> trace_shadow
 //
-// A path's state lives in SoA arrays indexed by its slot.  All paths that are
-// alive at bounce k have depth k, so depth is the loop index.  The L update
-// order of the reference (emission MIS, then NEE, per bounce) is kept because
-// trace_shadow of bounce k runs before shade of bounce k+1.  Radiance of each
-// (pixel, sample) is kept per slot and folded into the film in sample order by
-// k_film, which makes the film bit-identical to ProgressiveIntegrator.RenderTile
-// (Base/Integrator.fs:36-44).
+// first retires the paths that ended in the previous iteration and refills
+// their slots with new camera paths (regen), then advances every active path
+// by one bounce whatever its depth (the depth travels with the path).  The
+// GPU therefore stays full until the last paths of the round drain, instead
+// of idling through the long tail of every bounce loop.  The L update order
+// of the reference (emission MIS, then NEE, per bounce) is kept because
+// trace_shadow of bounce k runs before shade of bounce k+1.  The radiance of
+// every (pixel, sample) is stored by path id when the path retires and folded
+// into the film in sample order by k_film, which makes the film bit-identical
+// to ProgressiveIntegrator.RenderTile (Base/Integrator.fs:36-44) for any
+// schedule.
 //
 // Every kernel exists in two modes.  Plain: one scene and one film (kernel
-// arguments).  Batch: the slots of one round belong to several render items
+// arguments).  Batch: the paths of one round belong to several render items
 // (scene, film, depth limits, camera) of a device-side round table; each slot
 // records its item, and each lane reads its own scene.  Per-path random
 // streams do not depend on scheduling, so an item's film is bit-identical in
@@ -62,11 +67,11 @@ struct Work<false> {
     DevScene S;
     RoundItem it;
     __device__ __forceinline__ int n() const { return 1; }
-    __device__ __forceinline__ int item_of_new_slot(uint32_t) const { return 0; }
+    __device__ __forceinline__ int item_of_path(uint32_t) const { return 0; }
     __device__ __forceinline__ int item_of(const PathBuf&, uint32_t) const { return 0; }
     __device__ __forceinline__ const RoundItem& item(int) const { return it; }
     __device__ __forceinline__ const DevScene& scene(int) const { return S; }
-    __device__ __forceinline__ uint32_t total_slots() const { return (uint32_t)it.slot_count; }
+    __device__ __forceinline__ uint32_t total_paths() const { return (uint32_t)it.path_count; }
 };
 
 template <>
@@ -75,23 +80,23 @@ struct Work<true> {
     const RoundItem* items;
     const DevScene* scenes;
     __device__ __forceinline__ int n() const { return hdr->n_items; }
-    // item owning slot (the round table is ordered by slot_base)
-    __device__ __forceinline__ int item_of_new_slot(uint32_t slot) const {
+    // item owning a path id (the round table is ordered by path_base)
+    __device__ __forceinline__ int item_of_path(uint32_t id) const {
         int lo = 0, hi = hdr->n_items - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if ((uint32_t)items[mid].slot_base <= slot) lo = mid; else hi = mid - 1;
+            if ((uint32_t)items[mid].path_base <= id) lo = mid; else hi = mid - 1;
         }
         return lo;
     }
     __device__ __forceinline__ int item_of(const PathBuf& P, uint32_t slot) const { return P.item[slot]; }
     __device__ __forceinline__ const RoundItem& item(int k) const { return items[k]; }
     __device__ __forceinline__ const DevScene& scene(int k) const { return scenes[items[k].scene]; }
-    __device__ __forceinline__ uint32_t total_slots() const { return hdr->n_slots; }
+    __device__ __forceinline__ uint32_t total_paths() const { return hdr->n_paths; }
 };
 
-// local slot -> (pixel, sample): 8x8 pixel tiles, sample-major.
-__device__ __forceinline__ bool slot_pixel(const RoundItem& c, uint32_t q_all, int& x, int& y, int& s) {
+// local path id -> (pixel, sample): 8x8 pixel tiles, sample-major.
+__device__ __forceinline__ bool path_pixel(const RoundItem& c, uint32_t q_all, int& x, int& y, int& s) {
     const uint32_t q = q_all % (uint32_t)c.p_pad;
     s = c.s0 + (int)(q_all / (uint32_t)c.p_pad);
     const uint32_t tile = q >> 6;
@@ -100,75 +105,152 @@ __device__ __forceinline__ bool slot_pixel(const RoundItem& c, uint32_t q_all, i
     return x < c.width && y < c.height;
 }
 
-// ------------------------------------------------------------------ raygen
-// RenderTile seeding and primary ray (Base/Integrator.fs:34-39).
+// ------------------------------------------------------------------- regen
+// Free slot j first retires its finished path (radiance stored by path id),
+// then takes path id next_pid + j of the round: RenderTile seeding and the
+// primary ray (Base/Integrator.fs:34-39), appended to the active queue after
+// the paths that continue.  Ids and queue positions follow from j, so the
+// kernel needs no atomics; k_prep then advances the counters.  A padding
+// pixel, or a path with max-depth 0 (radiance 0), becomes a placeholder with
+// depth -1 that misses at once.  Past the last id the slot goes idle.
+__device__ __forceinline__ uint32_t new_paths(uint32_t n_free, uint32_t next_pid, uint32_t n_paths) {
+    // signed 64-bit: the difference can neither wrap nor be assumed not to
+    const int64_t left = (int64_t)n_paths - (int64_t)next_pid;
+    return left <= 0 ? 0u : (uint32_t)(left < (int64_t)n_free ? left : (int64_t)n_free);
+}
+
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_raygen(Work<BATCH> W, PathBuf P, Ctl* ctl, uint32_t* queue) {
-    const uint32_t n = W.total_slots();
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t slot = base + threadIdx.x;
-        int x = 0, y = 0, s = 0;
-        bool ok = false;
-        if (slot < n) {
-            const int k = W.item_of_new_slot(slot);
-            const RoundItem& it = W.item(k);
-            ok = slot_pixel(it, slot - (uint32_t)it.slot_base, x, y, s);
-            if (ok) {
-                const DevScene& S = W.scene(k);
-                uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(it.frame_id * it.spp + s));
-                V2 uPix, uLens;
-                uPix.x = lcg(rng); uPix.y = lcg(rng);
-                uLens.x = lcg(rng); uLens.y = lcg(rng);
-                Ray r = camera_ray(S, x, y, uPix, uLens);
-                P.o[slot] = f4(r.o, 0.0f);
-                P.d[slot] = f4(r.d, 0.0f);
-                P.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-                P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                P.rng[slot] = rng;
-                if (BATCH) P.item[slot] = k;
-            }
+__global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const Ctl* ctl, int cur,
+                                               const uint32_t* freel_in, uint32_t* queue, float4* L_out) {
+    const uint32_t n_free = ctl->free_count[cur];
+    const uint32_t next_pid = ctl->next_pid;
+    const uint32_t n_new = new_paths(n_free, next_pid, W.total_paths());
+    uint32_t* q = queue + ctl->q_count[cur];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_free; j += gridDim.x * blockDim.x) {
+        const uint32_t slot = freel_in[j];
+        const uint32_t old = P.pid[slot];
+        if (old != kNoPath) L_out[old] = P.L[slot];
+        if (j >= n_new) {
+            P.pid[slot] = kNoPath;
+            continue;
         }
-        wave_push(ok, &ctl->q_count[0], queue, slot);
+        const uint32_t id = next_pid + j;
+        const int k = W.item_of_path(id);
+        const RoundItem& it = W.item(k);
+        int x = 0, y = 0, s = 0;
+        const bool ok = path_pixel(it, id - (uint32_t)it.path_base, x, y, s) && it.max_depth > 0;
+        if (ok) {
+            const DevScene& S = W.scene(k);
+            uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(it.frame_id * it.spp + s));
+            V2 uPix, uLens;
+            uPix.x = lcg(rng); uPix.y = lcg(rng);
+            uLens.x = lcg(rng); uLens.y = lcg(rng);
+            Ray r = camera_ray(S, x, y, uPix, uLens);
+            P.o[slot] = f4(r.o, __int_as_float(0));
+            P.d[slot] = f4(r.d, 0.0f);
+            P.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            P.rng[slot] = rng;
+        } else {
+            P.o[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+            P.d[slot] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        }
+        P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.pid[slot] = id;
+        if (BATCH) P.item[slot] = k;
+        q[j] = slot;
     }
 }
 
-// Counter clear at the start of a bounce loop, and per-bounce reset (kernels,
+// After regen: the new paths join the active queue, the ids advance, and the
+// counters of the other parity (filled during this iteration) start at zero.
+template <bool BATCH>
+__global__ void k_prep(Work<BATCH> W, Ctl* ctl, int cur, uint32_t* fetch_seg) {
+    const int i = threadIdx.x;
+    const int nxt = 1 - cur;
+    if (i < 2 * kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
+    if (i < kBins) ctl->bin_count[i] = 0;
+    if (i == 0) {
+        const uint32_t n_new = new_paths(ctl->free_count[cur], ctl->next_pid, W.total_paths());
+        ctl->q_count[cur] += n_new;
+        ctl->next_pid += n_new;
+        ctl->sh_count = 0;
+        ctl->q_count[nxt] = 0;
+        ctl->free_count[nxt] = 0;
+    }
+}
+
+// Start of a round: every slot free and idle.
+__global__ void k_init_slots(PathBuf P, Ctl* ctl, uint32_t* freel, uint32_t n_slots) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_slots; j += gridDim.x * blockDim.x) {
+        freel[j] = j;
+        P.pid[j] = kNoPath;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->free_count[0] = n_slots;
+}
+
+// Counter clear at the start of a round, and per-iteration reset (kernels,
 // so that a captured graph carries no memset nodes).
 __global__ void k_ctl_clear(Ctl* ctl) {
     uint32_t* w = reinterpret_cast<uint32_t*>(ctl);
     if (threadIdx.x < sizeof(Ctl) / 4) w[threadIdx.x] = 0;
 }
-__global__ void k_reset(Ctl* ctl, int nxt) {
+__global__ void k_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg) {
     const int i = threadIdx.x;
+    if (i < 2 * kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
     if (i < kBins) ctl->bin_count[i] = 0;
-    if (i == 0) { ctl->sh_count = 0; ctl->q_count[nxt] = 0; }
+    if (i == 0) { ctl->sh_count = 0; ctl->q_count[nxt] = 0; ctl->free_count[nxt] = 0; }
     if (i < 4) ctl->fetch[i] = 0;
 }
 
 // ------------------------------------------------------------ trace closest
 // Lane-refilling persistent traversal: a wave keeps up to 64 rays in flight,
 // advances each by one item per iteration and refills idle lanes from the
-// queue once at least `refill` of them are idle (one atomic per wave).
+// queue once at least `refill` of them are idle.  The queue is cut into
+// segments with one fetch counter each (a counter serialises its atomics, so
+// one shared counter would cap the whole GPU at one refill per ~11 ns): a
+// wave starts on its own segment and moves on when that one runs dry.
+struct SegFetch {
+    uint32_t* ctr;    // segs counters, kSegStride words apart, zero at launch
+    int segs;
+};
+
 template <bool ANY, bool COUNT, int K, typename Init, typename Done>
-__device__ __forceinline__ void persistent_trace(uint32_t* fetch_ctr, uint32_t count, uint32_t* stk, uint32_t* spill,
+__device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uint32_t* stk, uint32_t* spill,
                                                  size_t spill_stride, uint32_t refill, TraceCount& tc, Init init,
                                                  Done done) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     Traversal<K> tr;
     tr.st = LaneStack<K>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
-    bool active = false, more = true;
+    const uint32_t seg_len = (count + (uint32_t)F.segs - 1) / (uint32_t)F.segs;
+    int seg = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (uint32_t)F.segs);
+    int tried = 0;
+    bool active = false, more = count > 0;
     uint32_t item = 0;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (more && n_idle >= refill) {
-            const uint32_t base = wave_fetch(fetch_ctr, n_idle);
-            if (base + n_idle >= count) more = false;
+            const uint32_t lo = (uint32_t)seg * seg_len;
+            const uint32_t hi = min(count, lo + seg_len);
+            uint32_t* c = F.ctr + (size_t)seg * kSegStride;
+            uint32_t b = 0;
+            if (lane_id() == 0) {
+                const uint32_t seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                b = lo + seen < hi ? atomicAdd(c, n_idle) : seg_len;
+            }
+            b = __shfl(b, 0);
+            // idle lanes take [start, start + n_idle) clipped to the segment;
+            // each lane bounds its own item (start may lie past hi)
+            const uint32_t start = lo + b;
+            if (start + n_idle >= hi) {
+                seg = seg + 1 == F.segs ? 0 : seg + 1;
+                if (++tried >= F.segs) more = false;
+            }
             if (!active) {
-                const uint32_t i = base + mbcnt(idle);
-                if (i < count) {
+                const uint32_t i = start + mbcnt(idle);
+                if (i < hi) {
                     item = i;
-                    init(tr, i);
+                    init(tr, item);
                     active = true;
                 }
             }
@@ -192,15 +274,18 @@ __device__ __forceinline__ void persistent_trace(uint32_t* fetch_ctr, uint32_t c
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* spill, size_t spill_stride,
-                                                       uint32_t refill, unsigned long long* stats) {
+                                                       uint32_t refill, unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
     const uint32_t count = ctl->q_count[cur];
     TraceCount tc{0, 0, 0};
     persistent_trace<false, COUNT, K>(
-        &ctl->fetch[0], count, stk, spill, spill_stride, refill, tc,
+        SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = queue[i];
-            tr.start(W.scene(W.item_of(P, slot)), {xyz(P.o[slot]), xyz(P.d[slot])}, kInf);
+            const float4 o4 = P.o[slot];
+            tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.d[slot])}, kInf);
+            // a placeholder path (padding pixel, max-depth 0) misses at once
+            if (__float_as_int(o4.w) < 0) tr.st.sp = 0;
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool) {
             if (fin) {
@@ -224,16 +309,19 @@ __global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P,
 // Material binning of the closest hits (sorting the shade queue by material
 // kind).  A block bins kBinItems consecutive queue entries: positions inside
 // the block come from wave-aggregated LDS counters, and each bin costs one
-// global atomic per block.
+// global atomic per block.  A miss ends the path: with a free list (freel
+// non-null) its slot goes there, as one more bin; the PSSMLT driver passes
+// no list and retires misses itself.
 constexpr int kBinPer = 16;
 constexpr int kBinItems = 256 * kBinPer;
 
 __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins,
-                                             uint32_t bin_cap) {
-    __shared__ uint32_t cnt[kBins], gbase[kBins];
+                                             uint32_t bin_cap, uint32_t* freel) {
+    __shared__ uint32_t cnt[kBins + 1], gbase[kBins + 1];
     const uint32_t count = ctl->q_count[cur];
+    const int n_out = freel ? kBins + 1 : kBins;
     for (uint32_t blk = blockIdx.x * (uint32_t)kBinItems; blk < count; blk += gridDim.x * (uint32_t)kBinItems) {
-        if (threadIdx.x < kBins) cnt[threadIdx.x] = 0;
+        if (threadIdx.x <= kBins) cnt[threadIdx.x] = 0;
         __syncthreads();
         uint32_t slot[kBinPer], pos[kBinPer];
         int bin[kBinPer];
@@ -243,10 +331,11 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
             slot[k] = 0;
             if (i < count) {
                 slot[k] = queue[i];
-                bin[k] = P.hit[slot[k]].w;
+                const int b = P.hit[slot[k]].w;
+                bin[k] = b >= 0 ? b : (freel ? kBins : -1);
             }
             pos[k] = 0;
-            for (int b = 0; b < kBins; ++b) {
+            for (int b = 0; b < n_out; ++b) {
                 const uint64_t m = __ballot(bin[k] == b);
                 if (m == 0) continue;
                 const int leader = __ffsll((unsigned long long)m) - 1;
@@ -257,12 +346,47 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
             }
         }
         __syncthreads();
-        if (threadIdx.x < kBins) gbase[threadIdx.x] = atomicAdd(&ctl->bin_count[threadIdx.x], cnt[threadIdx.x]);
+        if (threadIdx.x < (uint32_t)n_out && cnt[threadIdx.x]) {
+            uint32_t* ctr = threadIdx.x < kBins ? &ctl->bin_count[threadIdx.x] : &ctl->free_count[1 - cur];
+            gbase[threadIdx.x] = atomicAdd(ctr, cnt[threadIdx.x]);
+        }
         __syncthreads();
-        for (int k = 0; k < kBinPer; ++k)
-            if (bin[k] >= 0) bins[(size_t)bin[k] * bin_cap + gbase[bin[k]] + pos[k]] = slot[k];
+        for (int k = 0; k < kBinPer; ++k) {
+            if (bin[k] < 0) continue;
+            if (bin[k] < kBins) bins[(size_t)bin[k] * bin_cap + gbase[bin[k]] + pos[k]] = slot[k];
+            else freel[gbase[kBins] + pos[k]] = slot[k];
+        }
         __syncthreads();
     }
+}
+
+// Block-aggregated append to N queues: one global atomic per queue and block.
+// Every thread of the (256-thread) block must call it.
+template <int N>
+__device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* const (&ctr)[N],
+                                           uint32_t* const (&arr)[N], uint32_t val) {
+    __shared__ uint32_t wcnt[N][5];
+    const int w = (int)(threadIdx.x >> 6);
+    uint64_t m[N];
+    for (int o = 0; o < N; ++o) {
+        m[o] = __ballot(pred[o]);
+        if (lane_id() == 0) wcnt[o][w] = (uint32_t)__popcll(m[o]);
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)N) {
+        const int o = (int)threadIdx.x;
+        uint32_t run = 0;
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t c = wcnt[o][v];
+            wcnt[o][v] = run;
+            run += c;
+        }
+        wcnt[o][4] = run ? atomicAdd(ctr[o], run) : 0u;
+    }
+    __syncthreads();
+    for (int o = 0; o < N; ++o)
+        if (pred[o]) arr[o][wcnt[o][4] + wcnt[o][w] + mbcnt(m[o])] = val;
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------- shade
@@ -270,17 +394,19 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
 // hit: emission with MIS (:30-40), next-event estimation set-up (:42-50), BSDF
 // sampling (:61-67) and Russian roulette (:69-75).  The NEE contribution is
 // applied by k_trace_shadow when the shadow ray is unoccluded (:47-59).
+// The path's bounce depth is o.w; a path that does not continue frees its
+// slot for the next iteration (its NEE term still lands in this iteration).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* bins,
                                                uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                                               int depth) {
+                                               uint32_t* freel) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
     const uint32_t total = start[kBins];
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, shadow = false;
+        bool cont = false, shadow = false, fin = false;
         uint32_t slot = 0;
         if (i < total) {
             int b = 0;
@@ -292,6 +418,7 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
             const float4 o4 = P.o[slot], d4 = P.d[slot];
             const Ray r = {xyz(o4), xyz(d4)};
             const float prev_pdf = d4.w;
+            const int depth = __float_as_int(o4.w);
             const int4 h = P.hit[slot];
             const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
             const int4 info = S.insts[h.x].info;
@@ -336,7 +463,7 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
                         if (lcg(rng) < q) beta = beta * rcp_e(q);
                         else cont = false;
                     }
-                    P.o[slot] = f4(g.p, 0.0f);
+                    P.o[slot] = f4(g.p, __int_as_float(depth + 1));
                     P.d[slot] = f4(wi, bs.pdf);
                     P.beta[slot] = f4(beta, 0.0f);
                 }
@@ -344,9 +471,10 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
                 cont = cont && depth + 1 < it.max_depth;
             }
             P.L[slot] = f4(L, 0.0f);
+            fin = !cont;
         }
-        wave_push(cont, &ctl->q_count[nxt], queue_next, slot);
-        wave_push(shadow, &ctl->sh_count, shq, slot);
+        block_push<3>({cont, shadow, fin}, {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
+                      {queue_next, shq, freel}, slot);
     }
 }
 
@@ -356,12 +484,12 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
-                                                      unsigned long long* stats) {
+                                                      unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT, K>(
-        &ctl->fetch[1], count, stk, spill, spill_stride, refill, tc,
+        SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
@@ -389,7 +517,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, 
 // thread handles one pixel of one item of the round (items are ordered by
 // their cumulative pixel end pix_end).
 template <bool BATCH>
-__global__ void k_film(Work<BATCH> W, PathBuf P, float* film) {
+__global__ void k_film(Work<BATCH> W, const float4* L_out, float* film) {
     const int n_items = W.n();
     const uint32_t n_pix = (uint32_t)W.item(n_items - 1).pix_end;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += gridDim.x * blockDim.x) {
@@ -410,7 +538,7 @@ __global__ void k_film(Work<BATCH> W, PathBuf P, float* film) {
         float* f = film + c.film_offset + 3 * ((size_t)(c.height - y - 1) * c.width + x);
         float ax = f[0], ay = f[1], az = f[2];
         for (int s = 0; s < c.ns; ++s) {
-            const float4 L = P.L[(size_t)c.slot_base + (size_t)s * c.p_pad + q];
+            const float4 L = L_out[(size_t)c.path_base + (size_t)s * c.p_pad + q];
             ax = fmaf(c.inv_spp, L.x * rcp_e(1.0f), ax);
             ay = fmaf(c.inv_spp, L.y * rcp_e(1.0f), ay);
             az = fmaf(c.inv_spp, L.z * rcp_e(1.0f), az);
@@ -420,20 +548,26 @@ __global__ void k_film(Work<BATCH> W, PathBuf P, float* film) {
 }
 
 // NormalIntegrator.Li (Extensions/Integrator/Normal.fs:10-17) on the bins of
-// the first bounce: 0.5 * (n + 1).
-__global__ void k_shade_normal(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap) {
+// the first bounce: 0.5 * (n + 1); every path ends here.
+__global__ void k_shade_normal(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap, int nxt,
+                               uint32_t* freel) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
     const uint32_t total = start[kBins];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        int b = 0;
-        while (i >= start[b + 1]) ++b;
-        const uint32_t slot = bins[(size_t)b * bin_cap + (i - start[b])];
-        const Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
-        const int4 h = P.hit[slot];
-        const Geom g = hit_geometry(W.S, r, h.x, h.y, __int_as_float(h.z));
-        P.L[slot] = f4(0.5f * (g.n + mk(1, 1, 1)), 0.0f);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t slot = 0;
+        if (i < total) {
+            int b = 0;
+            while (i >= start[b + 1]) ++b;
+            slot = bins[(size_t)b * bin_cap + (i - start[b])];
+            const Ray r = {xyz(P.o[slot]), xyz(P.d[slot])};
+            const int4 h = P.hit[slot];
+            const Geom g = hit_geometry(W.S, r, h.x, h.y, __int_as_float(h.z));
+            P.L[slot] = f4(0.5f * (g.n + mk(1, 1, 1)), 0.0f);
+        }
+        block_push<1>({i < total}, {&ctl->free_count[nxt]}, {freel}, slot);
     }
 }
 
@@ -445,7 +579,7 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
     const uint32_t n = (uint32_t)width * (uint32_t)height;
     TraceCount tc{0, 0, 0};
     persistent_trace<false, false, kStackK>(
-        fetch, n, stk, spill, spill_stride, 16u, tc,
+        SegFetch{fetch, 1}, n, stk, spill, spill_stride, 16u, tc,
         [&](Traversal<kStackK>& tr, uint32_t p) {
             const int x = (int)(p % (uint32_t)width), y = (int)(p / (uint32_t)width);
             uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
@@ -484,7 +618,7 @@ static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const Path
                              const uint32_t* queue, const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
 #define BN_TC(C, KK)                                                                                            \
     hipLaunchKernelGGL((k_trace_closest<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, cur, queue, \
-                       B.spill, B.spill_stride, refill, B.stats)
+                       B.spill, B.spill_stride, refill, B.stats, B.fetch_seg)
     if (k == 16) { if (count) BN_TC(true, 16); else BN_TC(false, 16); }
     else { if (count) BN_TC(true, 32); else BN_TC(false, 32); }
 #undef BN_TC
@@ -494,72 +628,95 @@ static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathB
                             const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
 #define BN_TS(C, KK)                                                                                              \
     hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, B.spill, \
-                       B.spill_stride, refill, B.stats)
+                       B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
     if (k == 16) { if (count) BN_TS(true, 16); else BN_TS(false, 16); }
     else { if (count) BN_TS(true, 32); else BN_TS(false, 32); }
 #undef BN_TS
 }
 
+struct Bracket {
+    LaunchHook* h;
+    hipStream_t s;
+    Bracket(LaunchHook* hk, const char* name, hipStream_t st_) : h(hk), s(st_) { if (h) h->begin(name, s); }
+    ~Bracket() { if (h) h->end(s); }
+};
+
 template <bool BATCH>
-static void enqueue_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
-                      int depth_n, bool count, LaunchHook* hook) {
+static void round_begin_t(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, LaunchHook* hook) {
     hipStream_t st = L.stream;
-    struct Bracket {
-        LaunchHook* h;
-        hipStream_t s;
-        Bracket(LaunchHook* hk, const char* name, hipStream_t st_) : h(hk), s(st_) { if (h) h->begin(name, s); }
-        ~Bracket() { if (h) h->end(s); }
-    };
     { Bracket b(hook, "ctl_clear", st); hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl); }
-    { Bracket b(hook, "raygen", st);
-      hipLaunchKernelGGL(k_raygen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.queue[0]); }
-    int cur = 0;
-    for (int depth = 0; depth < depth_n; ++depth) {
-        const int nxt = 1 - cur;
-        { Bracket b(hook, "reset", st); hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt); }
-        { Bracket b(hook, "trace_closest", st); launch_closest_t<BATCH>(count, B.stack_k, W, P, ctl, cur, B.queue[cur],
-                                                                         B, L.refill, L.grid_trace, st); }
+    { Bracket b(hook, "init_slots", st);
+      hipLaunchKernelGGL(k_init_slots, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, B.freel[0], L.n_slots); }
+}
+
+template <bool BATCH>
+static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl,
+                         const Buffers& B, int n, bool count, LaunchHook* hook) {
+    hipStream_t st = L.stream;
+    for (int i = 0; i < n; ++i) {
+        const int cur = i & 1, nxt = 1 - cur;
+        { Bracket b(hook, "regen", st);
+          hipLaunchKernelGGL(k_regen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, cur, B.freel[cur],
+                             B.queue[cur], B.L_out); }
+        { Bracket b(hook, "prep", st);
+          hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg); }
+        { Bracket b(hook, "trace_closest", st);
+          launch_closest_t<BATCH>(count, B.stack_k, W, P, ctl, cur, B.queue[cur], B, L.refill, L.grid_trace, st); }
         { Bracket b(hook, "bin", st);
           hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, cur, B.queue[cur], B.bins,
-                             B.bin_cap); }
-        { Bracket b(hook, "shade", st);
-          hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
-                             B.queue[nxt], nxt, B.shq, depth); }
-        { Bracket b(hook, "trace_shadow", st); launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, B, L.refill,
-                                                                       L.grid_trace, st); }
+                             B.bin_cap, B.freel[nxt]); }
+        if (kind == Integrator::Normal) {
+            if constexpr (!BATCH) {
+                Bracket b(hook, "shade_normal", st);
+                hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
+                                   nxt, B.freel[nxt]);
+            }
+        } else {
+            { Bracket b(hook, "shade", st);
+              hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
+                                 B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
+            { Bracket b(hook, "trace_shadow", st);
+              launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, B, L.refill, L.grid_trace, st); }
+        }
         if (hook) hook->after_bounce(cur);
-        cur = nxt;
     }
-    { Bracket b(hook, "film", st);
-      hipLaunchKernelGGL(k_film<BATCH>, dim3(BATCH ? L.grid_shade : (L.n_pix + 255) / 256), dim3(256), 0, st, W, P,
-                         B.film); }
 }
 
-void enqueue_bounce_loop(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, int depth_n,
-                         bool count, LaunchHook* hook) {
-    if (batch) enqueue_t<true>(batch_work(L), L, P, ctl, B, depth_n, count, hook);
-    else enqueue_t<false>(plain_work(L), L, P, ctl, B, depth_n, count, hook);
-}
-
-void enqueue_normal(const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B) {
-    const Work<false> W = plain_work(L);
+template <bool BATCH>
+static void round_end_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                        LaunchHook* hook) {
     hipStream_t st = L.stream;
-    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
-    hipLaunchKernelGGL(k_raygen<false>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.queue[0]);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, 1);
-    launch_closest_t<false>(false, B.stack_k, W, P, ctl, 0, B.queue[0], B, L.refill, L.grid_trace, st);
-    hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, 0, B.queue[0], B.bins, B.bin_cap);
-    hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap);
-    hipLaunchKernelGGL(k_film<false>, dim3((L.n_pix + 255) / 256), dim3(256), 0, st, W, P, B.film);
+    // iteration blocks are even: the last paths sit in free list 0
+    { Bracket b(hook, "regen", st);
+      hipLaunchKernelGGL(k_regen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, 0, B.freel[0],
+                         B.queue[0], B.L_out); }
+    { Bracket b(hook, "film", st);
+      hipLaunchKernelGGL(k_film<BATCH>, dim3(BATCH ? L.grid_shade : (L.n_pix + 255) / 256), dim3(256), 0, st, W,
+                         B.L_out, B.film); }
+}
+
+void enqueue_round_begin(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                         LaunchHook* hook) {
+    if (batch) round_begin_t<true>(L, P, ctl, B, hook);
+    else round_begin_t<false>(L, P, ctl, B, hook);
+}
+void enqueue_iterations(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                        int n, bool count, LaunchHook* hook) {
+    if (batch) iterations_t<true>(batch_work(L), kind, L, P, ctl, B, n, count, hook);
+    else iterations_t<false>(plain_work(L), kind, L, P, ctl, B, n, count, hook);
+}
+void enqueue_round_end(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, LaunchHook* hook) {
+    if (batch) round_end_t<true>(batch_work(L), L, P, ctl, B, hook);
+    else round_end_t<false>(plain_work(L), L, P, ctl, B, hook);
 }
 
 // Single kernels used by the PSSMLT driver (plain scene).
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap);
+                uint32_t* freel, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap, freel);
 }
-void launch_reset(Ctl* ctl, int nxt, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctl, nxt);
+void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(2 * kFetchSegs), 0, st, ctl, nxt, fetch_seg);
 }
 void launch_ctl_clear(Ctl* ctl, hipStream_t st) {
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);

@@ -289,7 +289,9 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
 /* Tunables and instrumentation (no reference counterpart).  Keys:
  *   "timing"               1: record a HIP event pair around every kernel
  *                          launch of the render calls, on the context's stream
- *   "chunk_slots"          path slots in flight per bounce loop (default 64M)
+ *   "chunk_slots"          path slots in flight (default 64M)
+ *   "window_paths"         paths per round, radiance kept by path id (default 0:
+ *                          as many as chunk_slots, every path in flight at once)
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
  *   "refill"               idle lanes of a wave that trigger a refill (1..64)
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass

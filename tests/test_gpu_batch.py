@@ -39,17 +39,24 @@ def test_batch_equals_plain(ctx, jobs, graph):
         assert np.array_equal(f.view(np.int32), plain.view(np.int32))
 
 
-def test_batch_many_rounds_equals_plain(ctx, jobs):
-    """A slot capacity smaller than one item's frame forces many rounds and
-    splits items across rounds; films stay bit-identical."""
-    ctx.set_option("chunk_slots", 4096)
+@pytest.mark.parametrize("window,slots", [(4096, 64 << 20), (1 << 20, 3000), (5000, 700)])
+def test_batch_many_rounds_equals_plain(ctx, jobs, window, slots):
+    """Small rounds split items across rounds, and a small slot pool makes
+    every slot carry many paths of several items; films stay bit-identical."""
+    ctx.set_option("window_paths", window)
+    ctx.set_option("chunk_slots", slots)
     try:
         films, st = ctx.render_batch([(ds, p) for ds, p, _ in jobs])
+        plain = [ds.render_pt(p) for ds, p, _ in jobs]
     finally:
+        ctx.set_option("window_paths", 0)
         ctx.set_option("chunk_slots", 64 << 20)
-    assert st["rounds"] > 3
-    for (ds, p, _), f in zip(jobs, films):
-        assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
+    if window < (1 << 20):
+        assert st["rounds"] > 3
+    for (ds, p, _), f, g in zip(jobs, films, plain):
+        ref = ds.render_pt(p)
+        assert np.array_equal(f.view(np.int32), ref.view(np.int32))
+        assert np.array_equal(g.view(np.int32), ref.view(np.int32))
 
 
 @pytest.mark.parametrize("i", [0, 1, 2, 3, 4, 5])

@@ -69,3 +69,25 @@ def test_trace_counters_match_oracle(ctx, orc, c0):
     for k in ("rays_closest", "rays_shadow", "nodes_closest", "tris_closest", "insts_closest",
               "nodes_shadow", "tris_shadow", "insts_shadow", "vertices", "paths"):
         assert gs[k] == os_[k], k
+
+
+def test_normal_view_config1(ctx, orc, c1_small):
+    """NormalIntegrator (Extensions/Integrator/Normal.fs:10-17): one bounce."""
+    p = c1_small.pt_params(spp=3)
+    g = ctx.scene(c1_small).render_normal(p)
+    o, _ = orc.scene(c1_small).render(p, integrator=0)
+    assert np.mean(g == o) > 0.999
+    assert rel_rmse(g, o) <= 1e-5
+
+
+@pytest.mark.parametrize("depth", [0, 1, 3])
+def test_pt_shallow_depths(ctx, orc, c0, depth):
+    """max-depth 0 renders black; 1 is emission only; paths of different
+    depths share the slot pool."""
+    p = c0.pt_params(width=40, height=36, spp=3, max_depth=depth)
+    g = ctx.scene(c0).render_pt(p)
+    o, _ = orc.scene(c0).render(p)
+    if depth == 0:
+        assert not np.any(g)
+    assert np.mean(g == o) > 0.999
+    assert rel_rmse(g, o) <= 1e-3 if np.any(o) else not np.any(g)
