@@ -7,10 +7,18 @@ already resident in HBM.  Prints one JSON line:
   value          Mrays/s = (extension + shadow rays of one frame) * steps / time
   roofline       dominant kernel (trace_closest): algorithmic bytes per launch
                  (48 per ray + 32 per node + 36 per triangle + 48 per instance
-                 visited) / HIP-event launch duration, against 8 TB/s HBM
+                 visited) / HIP-event launch duration, against 8 TB/s HBM;
+                 traffic = HBM bytes per launch from the committed rocprofv3
+                 FETCH_SIZE / WRITE_SIZE summary
+  batched        32 seeded scenes rendered one after the other (plain) and as
+                 one batch; speedup = plain time / batched time
   cpu_baseline   the CPU oracle (C++ restatement of the F# renderer) on a
                  bounded pixel window of the same frame, all host threads up to 16
   parity         GPU vs oracle on a 128x128x2spp render of the same scene
+
+--copy-calibration additionally copies 1 GiB device to device (hipMemcpy) so
+that a rocprofv3 --pmc run of this script sees FETCH_SIZE / WRITE_SIZE of a
+copy of known size next to the trace kernels.
 """
 from __future__ import annotations
 
@@ -65,6 +73,49 @@ def cpu_baseline(sc, params, budget_s=12.0):
             "mpaths_per_s": paths / elapsed / 1e6}
 
 
+def batched(ctx, n=32):
+    """Plain per-scene renders against one batched render of the same items."""
+    t0 = time.perf_counter()
+    scs = [scene.load_scene(j) for j in configs.config2(n)]
+    ds = [ctx.scene(s) for s in scs]
+    jobs = [(d, s.pt_params()) for d, s in zip(ds, scs)]
+    setup = time.perf_counter() - t0
+    for d, p in jobs:
+        d.render_pt(p)
+    t0 = time.perf_counter()
+    plain = [d.render_pt(p) for d, p in jobs]
+    plain_s = time.perf_counter() - t0
+    ctx.render_batch(jobs)
+    t0 = time.perf_counter()
+    films, st = ctx.render_batch(jobs)
+    batch_s = time.perf_counter() - t0
+    equal = all(np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(films, plain))
+    paths = sum(p.width * p.height * p.spp for _, p in jobs)
+    return {"config": f"config2: {n} seeded scenes, 1-256 instances of 1k-500k-triangle displaced icospheres, "
+                      "256^2-1024^2, 4-64 spp, depth 4-16",
+            "paths": paths, "plain_s": plain_s, "batched_s": batch_s, "speedup": plain_s / batch_s,
+            "batched_mpaths_per_s": paths / batch_s / 1e6, "rounds": st["rounds"], "graph": st["graph"],
+            "films_bit_equal": equal, "scene_setup_s": setup}
+
+
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device hipMemcpy (the runtime's copy kernel) of nbytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:      # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    print(f"# copy calibration: {nbytes} bytes device to device", file=sys.stderr, flush=True)
+
+
 def parity(ctx):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -84,6 +135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--spp", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -128,7 +181,7 @@ def main():
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": "Mrays/s (extension + shadow rays), 1024x1024x64spp displaced-sphere Cornell box",
+        "metric": "Mrays/s (batched vs plain, %HBM peak) + adjoint iters/s at RMSE<=1e-3 vs CPU",
         "value": rays * args.steps / elapsed / 1e6,
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -153,8 +206,12 @@ def main():
                      "launches": launches},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in kt.items()},
     }
+    if not args.no_batch:
+        out["batched"] = batched(ctx)
     if not args.no_parity:
         out["parity"] = parity(ctx)
+    if args.copy_calibration:
+        copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, params)
     print(json.dumps(out), flush=True)
