@@ -1287,6 +1287,16 @@ int orc_trace_sample(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y,
     return 0;
 }
 
+int orc_camera_ray(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t k, float* out) {
+    Sampler smp{xx3((uint32_t)x, (uint32_t)y, (uint32_t)(p->frame_id * p->spp + k))};
+    V2 uPix = smp.next2();
+    V2 uLens = smp.next2();
+    Ray r = generatePrimary(*s, p->width, p->height, x, y, uPix, uLens);
+    out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
+    out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+    return 0;
+}
+
 int orc_first_hit(orc_scene* s, const orc_pt_params* p, int32_t k, int32_t* inst, int32_t* prim, float* tout) {
     for (int y = 0; y < p->height; ++y)
         for (int x = 0; x < p->width; ++x) {

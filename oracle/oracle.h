@@ -107,6 +107,10 @@ int orc_render(orc_scene* s, int32_t integrator, const orc_pt_params* p, int32_t
 int orc_trace_sample(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t sample_id,
                      float* rgb);
 
+/* Camera ray of sample sample_id of pixel (x,y) (Integrator.fs:35-39,
+ * Camera.fs:12-20): origin in out[0..2], direction in out[3..5]. */
+int orc_camera_ray(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t sample_id, float* out6);
+
 /* First hit of sample sample_id of every pixel (pixel order y*W+x). */
 int orc_first_hit(orc_scene* s, const orc_pt_params* p, int32_t sample_id,
                   int32_t* inst, int32_t* prim, float* t);

@@ -63,6 +63,7 @@ class Oracle:
                                  C.POINTER(OracleStats)] + [C.c_int32] * 4
         L.orc_trace_sample.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, C.c_int32, C.c_int32, f32p]
         L.orc_first_hit.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p]
+        L.orc_camera_ray.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, C.c_int32, C.c_int32, f32p]
         L.orc_render_pssmlt.argtypes = [C.c_void_p, C.POINTER(abi.MLTParams), C.c_int32, f32p,
                                         C.POINTER(abi.MLTStats), C.c_int32, C.c_int32, f32p, i32p]
         L.orc_scene_export.argtypes = [C.c_void_p, C.POINTER(abi.BVHNode), i32p, i32p, i32p, f32p, f32p,
@@ -167,6 +168,11 @@ class OracleScene:
         out = np.zeros(3, np.float32)
         self.o._check(self.o.lib.orc_trace_sample(self.h, C.byref(params), x, y, k, _ptr(out, f32p)))
         return out
+
+    def camera_ray(self, params, x, y, sample_id=0):
+        out = np.zeros(6, np.float32)
+        self.o._check(self.o.lib.orc_camera_ray(self.h, C.byref(params), x, y, sample_id, _ptr(out, f32p)))
+        return out[:3], out[3:]
 
     def first_hit(self, params, sample_id=0):
         n = params.width * params.height
