@@ -272,7 +272,7 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
  * scene per Scene.Render).  Each item is one PathTracingIntegrator frame of
  * a scene from bn_scene_build / bn_upload_scene; all items advance through
  * one launch sequence per round (scenes, resolutions, spp and depths may
- * differ), with the bounce loop of a round captured once as a HIP graph and
+ * differ), with a block of iterations captured once as a HIP graph and
  * replayed.  films[k] receives item k's film exactly as bn_render_pt would
  * write it (bit-identical); NULL entries are skipped. */
 typedef struct bn_batch_item {
