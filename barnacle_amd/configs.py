@@ -180,6 +180,83 @@ def config1(width=1024, height=1024, spp=64, max_depth=16, freq=59):
     return info
 
 
+def _morton3(q):
+    """Interleave three 10-bit integer coordinates (Morton order)."""
+    def spread(v):
+        v = v.astype(np.uint64) & 0x3FF
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        return v
+    return spread(q[:, 0]) | (spread(q[:, 1]) << 1) | (spread(q[:, 2]) << 2)
+
+
+def split_mesh(mesh, n_patches):
+    """Cut a mesh into n_patches spatially compact meshes of nearly equal
+    triangle count (Morton order of the triangle centroids)."""
+    V = np.asarray(mesh["vertices"], np.float32).reshape(-1, 3)
+    T = np.asarray(mesh["indices"], np.int64).reshape(-1, 3)
+    c = V[T].mean(axis=1)
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    q = np.floor((c - lo) / np.maximum(hi - lo, 1e-12) * 1023.0).astype(np.int64)
+    order = np.argsort(_morton3(q), kind="stable")
+    out = []
+    for part in np.array_split(order, n_patches):
+        tri = T[part]
+        used, inv = np.unique(tri.reshape(-1), return_inverse=True)
+        out.append({"type": "mesh", "vertices": V[used].reshape(-1), "indices": inv.reshape(-1).astype(np.int64)})
+    return out
+
+
+def config3(width=512, height=512, spp=32, max_depth=8, freq=59, n_patches=4096, seed=2):
+    """Inverse-rendering scene: config 1's box and displaced sphere, with the
+    sphere cut into n_patches instances, each with its own GGX material
+    (albedo rgb, roughness) and its own diffuse light (emission rgb), so the
+    uniform light sampler covers all of them.  Parameters are the ground
+    truth drawn from `seed`; config3_params gives them and the start point."""
+    info = cornell_box(width, height, spp, max_depth)
+    rng = np.random.default_rng(seed)
+    alb = rng.uniform(0.2, 0.9, (n_patches, 3)).astype(np.float32)
+    rough = rng.uniform(0.05, 0.8, n_patches).astype(np.float32)
+    em = rng.uniform(0.0, 0.2, (n_patches, 3)).astype(np.float32)
+    node = {"instances": [], "transform": len(info["transforms"])}
+    info["transforms"].append({"keyframes": [{"time": 0.0, "scale": [90.0] * 3,
+                                              "translation": [185.5, 300.0, -168.5]}]})
+    for k, prim in enumerate(split_mesh(displaced_sphere(freq, 0.08, 1), n_patches)):
+        info["primitives"].append(prim)
+        info["materials"].append({"type": "pbr", "albedo": [float(x) for x in alb[k]], "metallic": 0.0,
+                                  "roughness": float(rough[k])})
+        info["lights"].append({"type": "diffuse", "emission": [float(x) for x in em[k]]})
+        info["instances"].append({"primitive": len(info["primitives"]) - 1,
+                                  "material": len(info["materials"]) - 1, "light": len(info["lights"]) - 1})
+        node["instances"].append(len(info["instances"]) - 1)
+    info["nodes"].append(node)
+    info["nodes"][0]["children"].append(len(info["nodes"]) - 1)
+    info["x-patches"] = {"first_material": len(info["materials"]) - n_patches,
+                         "first_light": len(info["lights"]) - n_patches, "count": n_patches}
+    return info
+
+
+def config3_params(sc, info):
+    """(truth, start, mask): parameter arrays (material (n_mat, 4) albedo rgb +
+    roughness, light (n_light, 3) emission) of a loaded config-3 scene, the
+    start point (patch albedo 0.5, roughness 0.5, emission 0) and the mask
+    of the parameters being optimised (the patches)."""
+    mat = np.array([[*m["albedo"], m["roughness"]] for m in sc.materials], np.float32)
+    light = np.array([l["emission"] for l in sc.lights], np.float32)
+    p = info["x-patches"]
+    fm, fl, n = p["first_material"], p["first_light"], p["count"]
+    mat0, light0 = mat.copy(), light.copy()
+    mat0[fm:fm + n] = [0.5, 0.5, 0.5, 0.5]
+    light0[fl:fl + n] = 0.0
+    mmask = np.zeros(mat.shape, bool)
+    mmask[fm:fm + n] = True
+    lmask = np.zeros(light.shape, bool)
+    lmask[fl:fl + n] = True
+    return (mat, light), (mat0, light0), (mmask, lmask)
+
+
 def caustic_scene(width=1024, height=1024, spp=256, max_depth=16, freq=32,
                   n_bootstrap=1 << 20, n_chains=1 << 20, sigma=1.0 / 64.0, large_step=0.3):
     info = cornell_box(width, height, spp, max_depth, boxes=False)

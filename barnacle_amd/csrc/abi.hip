@@ -137,6 +137,32 @@ void validate_bvh(const std::vector<bn_bvh_node>& nodes, int n_items, const char
 }
 
 // Concatenate the referenced BLAS and upload the whole scene.
+// Device material records: (kind bits, albedo) (ior, alpha, metallic,
+// roughness).  PBRMaterial: Alpha = Max(roughness^2, 1e-3), Metallic =
+// Clamp(m, 0, 1) (PBR.fs:86-87); the raw roughness is kept for d alpha.
+std::vector<float4> pack_materials(const std::vector<bn_material>& mats) {
+    std::vector<float4> d(2 * std::max<size_t>(1, mats.size()));
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const bn_material& m = mats[i];
+        int kind = m.kind;
+        float kf;
+        std::memcpy(&kf, &kind, 4);
+        float r2 = m.roughness * m.roughness;
+        float alpha = (r2 != r2) ? r2 : (r2 > 1e-3f ? r2 : 1e-3f);
+        float met = m.metallic < 0.0f ? 0.0f : (m.metallic > 1.0f ? 1.0f : m.metallic);
+        d[2 * i] = make_float4(kf, m.albedo[0], m.albedo[1], m.albedo[2]);
+        d[2 * i + 1] = make_float4(m.ior, alpha, met, m.roughness);
+    }
+    return d;
+}
+std::vector<float4> pack_lights(const std::vector<bn_light>& lights) {
+    std::vector<float4> d(std::max<size_t>(1, lights.size()));
+    for (size_t i = 0; i < lights.size(); ++i)
+        d[i] = make_float4(lights[i].emission[0], lights[i].emission[1], lights[i].emission[2],
+                           lights[i].two_sided ? 1.0f : 0.0f);
+    return d;
+}
+
 void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
                   const std::vector<bn_instance_upload>& insts, const std::vector<bn_material>& mats,
                   const std::vector<bn_light>& lights, const bn_camera& cam, const float* c2w) {
@@ -194,24 +220,12 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
                            in.material, in.light);
         if (in.light >= 0) lrec.push_back(make_int4((int)i, d.info.y, ntri, in.light));
     }
-    std::vector<float4> dmats(2 * std::max<size_t>(1, mats.size()));
-    for (size_t i = 0; i < mats.size(); ++i) {
-        const bn_material& m = mats[i];
+    for (const auto& m : mats)
         if (m.kind < 0 || m.kind > 3) throw Error(BN_E_INVALID, "unknown material kind");
-        int kind = m.kind;
-        float kf;
-        std::memcpy(&kf, &kind, 4);
-        // PBRMaterial: Alpha = Max(roughness^2, 1e-3), Metallic = Clamp(m, 0, 1) (PBR.fs:86-87)
-        float r2 = m.roughness * m.roughness;
-        float alpha = (r2 != r2) ? r2 : (r2 > 1e-3f ? r2 : 1e-3f);
-        float met = m.metallic < 0.0f ? 0.0f : (m.metallic > 1.0f ? 1.0f : m.metallic);
-        dmats[2 * i] = make_float4(kf, m.albedo[0], m.albedo[1], m.albedo[2]);
-        dmats[2 * i + 1] = make_float4(m.ior, alpha, met, 0.0f);
-    }
-    std::vector<float4> dl(std::max<size_t>(1, lights.size()));
-    for (size_t i = 0; i < lights.size(); ++i)
-        dl[i] = make_float4(lights[i].emission[0], lights[i].emission[1], lights[i].emission[2],
-                            lights[i].two_sided ? 1.0f : 0.0f);
+    sc.mats_host = mats;
+    sc.lights_host = lights;
+    const std::vector<float4> dmats = pack_materials(mats);
+    const std::vector<float4> dl = pack_lights(lights);
     hipStream_t s = c->stream;
     sc.tlas.upload(tlas.data(), tlas.size() * sizeof(bn_bvh_node), s);
     sc.blas.upload(nodes.data(), nodes.size() * sizeof(bn_bvh_node), s);
@@ -405,7 +419,7 @@ void run_round(Ctx* c, bool batch, bnd::Integrator kind, const bnd::Launch& L, c
         if (h.next_pid >= n_paths && h.q_count[0] == 0) break;
         if (iters > limit) throw Error(BN_E_DEVICE, "round did not drain");
     }
-    bnd::enqueue_round_end(batch, L, P, ctl, B, hook);
+    bnd::enqueue_round_end(batch, kind, L, P, ctl, B, hook);
 }
 
 // One frame of PathTracingIntegrator (or the Normal view), in rounds of
@@ -467,6 +481,109 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
         acc.rays_closest -= (int64_t)p.spp * (it.p_pad - pix) + (p.max_depth <= 0 ? (int64_t)p.spp * pix : 0);
         *stats = acc;
     }
+}
+
+// Differentiable-rendering parameters: albedo and roughness of every
+// material, emission of every light definition.  Geometry, light records
+// and alias tables do not depend on them.
+void set_params(Ctx* c, SceneRec& sc, const float* mp, int n_mat, const float* le, int n_light) {
+    if (mp) {
+        if (n_mat != (int)sc.mats_host.size()) throw Error(BN_E_INVALID, "material parameter count mismatch");
+        for (int i = 0; i < n_mat; ++i) {
+            bn_material& m = sc.mats_host[i];
+            m.albedo[0] = mp[4 * i]; m.albedo[1] = mp[4 * i + 1]; m.albedo[2] = mp[4 * i + 2];
+            m.roughness = mp[4 * i + 3];
+        }
+        const std::vector<float4> d = pack_materials(sc.mats_host);
+        sc.mats.upload(d.data(), d.size() * sizeof(float4), c->stream);
+        sc.dev.mats = (const float4*)sc.mats.p;
+    }
+    if (le) {
+        if (n_light != (int)sc.lights_host.size()) throw Error(BN_E_INVALID, "light parameter count mismatch");
+        for (int i = 0; i < n_light; ++i)
+            for (int k = 0; k < 3; ++k) sc.lights_host[i].emission[k] = le[3 * i + k];
+        const std::vector<float4> d = pack_lights(sc.lights_host);
+        sc.lightdefs.upload(d.data(), d.size() * sizeof(float4), c->stream);
+        sc.dev.lightdefs = (const float4*)sc.lightdefs.p;
+    }
+    hip_check(hipStreamSynchronize(c->stream), "set params");
+    c->graphs.clear();
+}
+
+// Gradient of sum(d_film * film) for the frame bn_render_pt renders with the
+// same parameters: per round, the forward pass (radiance of every path id),
+// then the path replay (kernels_pt.hip, adjoint section).
+void backward_pt(Ctx* c, SceneRec& sc, const bn_pt_params& p, const float* d_film, float* d_mat, float* d_light) {
+    check_pt_params(p);
+    if (!d_film) throw Error(BN_E_INVALID, "null film adjoint");
+    if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
+    hipStream_t s = c->stream;
+    camera_constants(sc, p.width, p.height);
+    bnd::RoundItem it = plain_item(p);
+    const size_t want = std::max<size_t>((size_t)it.p_pad, window_paths(c));
+    const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / it.p_pad));
+    const size_t round_paths = (size_t)ns * it.p_pad;
+    const size_t slots = std::min(round_paths, c->chunk_slots);
+    ensure_paths(c, slots);
+    ensure_spill(c);
+    if (slots > c->adj_cap) {
+        for (DevBuf* b : {&c->a_delta, &c->a_pend_a, &c->a_pend_r, &c->a_nee_da, &c->a_nee_dr})
+            b->ensure(slots * sizeof(float4));
+        c->adj_cap = slots;
+    }
+    c->L_out.ensure(round_paths * sizeof(float4));
+    const size_t film_n = (size_t)p.width * p.height * 3;
+    const int n_mat = (int)sc.mats_host.size(), n_light = (int)sc.lights_host.size();
+    c->film.ensure(film_n * 4);
+    c->ctl.ensure(sizeof(bnd::Ctl));
+    c->stats.ensure(8 * sizeof(unsigned long long));
+    c->a_gmat.ensure((size_t)std::max(1, n_mat) * 16);
+    c->a_glight.ensure((size_t)std::max(1, n_light) * 16);
+    c->a_dfilm.upload(d_film, film_n * 4, s);
+    hip_check(hipMemsetAsync(c->film.p, 0, film_n * 4, s), "memset film");
+    hip_check(hipMemsetAsync(c->a_gmat.p, 0, (size_t)std::max(1, n_mat) * 16, s), "memset gradients");
+    hip_check(hipMemsetAsync(c->a_glight.p, 0, (size_t)std::max(1, n_light) * 16, s), "memset gradients");
+    bnd::PathBuf P = path_buf(c);
+    bnd::Ctl* ctl = (bnd::Ctl*)c->ctl.p;
+    Hook hook(c, ctl, nullptr);
+    bnd::Launch L{};
+    L.scene = sc.dev;
+    L.grid_trace = grid_trace(c);
+    L.grid_shade = c->n_cu * 8;
+    L.refill = (uint32_t)c->refill;
+    L.stream = s;
+    L.n_pix = (uint32_t)(p.width * p.height);
+    L.n_slots = (uint32_t)slots;
+    bnd::Buffers B = buffers(c);
+    B.adj.delta = (float4*)c->a_delta.p;
+    B.adj.pend_a = (float4*)c->a_pend_a.p;
+    B.adj.pend_r = (float4*)c->a_pend_r.p;
+    B.adj.nee_da = (float4*)c->a_nee_da.p;
+    B.adj.nee_dr = (float4*)c->a_nee_dr.p;
+    B.adj.L_fwd = (const float4*)c->L_out.p;
+    B.adj.d_film = (const float*)c->a_dfilm.p;
+    B.adj.g_mat = (float*)c->a_gmat.p;
+    B.adj.g_light = (float*)c->a_glight.p;
+    for (int s0 = 0; s0 < p.spp; s0 += ns) {
+        it.s0 = s0;
+        it.ns = std::min(ns, p.spp - s0);
+        it.path_count = it.ns * it.p_pad;
+        L.item = it;
+        run_round(c, false, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, false,
+                  &hook, nullptr);
+        run_round(c, false, bnd::Integrator::Adjoint, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, false,
+                  &hook, nullptr);
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+    std::vector<float> gm((size_t)std::max(1, n_mat) * 4), gl((size_t)std::max(1, n_light) * 4);
+    hip_check(hipMemcpyAsync(gm.data(), c->a_gmat.p, gm.size() * 4, hipMemcpyDeviceToHost, s), "gradient D2H");
+    hip_check(hipMemcpyAsync(gl.data(), c->a_glight.p, gl.size() * 4, hipMemcpyDeviceToHost, s), "gradient D2H");
+    hip_check(hipStreamSynchronize(s), "backward");
+    c->timer.resolve();
+    if (d_mat) std::memcpy(d_mat, gm.data(), (size_t)n_mat * 16);
+    if (d_light)
+        for (int i = 0; i < n_light; ++i)
+            for (int k = 0; k < 3; ++k) d_light[3 * i + k] = gl[4 * (size_t)i + k];
 }
 
 // Several renders in one launch sequence per round (see kernels_pt.hip).
@@ -1196,5 +1313,26 @@ extern "C" int bn_autotune(bn_ctx* ctx, const bn_batch_item* items, int32_t n, i
         if (!c) throw Error(BN_E_INVALID, "null context");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         autotune(c, items, n, flags, out);
+    });
+}
+
+extern "C" int bn_set_params(bn_ctx* ctx, int32_t scene, const float* mat_params, int32_t n_mat,
+                             const float* light_emission, int32_t n_light) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        SceneRec& sc = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        set_params(c, sc, mat_params, n_mat, light_emission, n_light);
+    });
+}
+
+extern "C" int bn_backward_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, const float* d_film,
+                              float* d_mat, float* d_light) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!params) throw Error(BN_E_INVALID, "null params");
+        SceneRec& sc = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        backward_pt(c, sc, *params, d_film, d_mat, d_light);
     });
 }

@@ -44,6 +44,8 @@ struct SceneRec {
     bn_camera camera{};
     float c2w[16] = {};
     bnd::DevScene dev{};
+    std::vector<bn_material> mats_host;     // what bn_set_params edits
+    std::vector<bn_light> lights_host;
     std::shared_ptr<bnh::HostScene> host;   // set by bn_scene_build
 };
 
@@ -83,6 +85,9 @@ struct Ctx {
     size_t path_cap = 0;
     DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b, p_item, p_pid;
     DevBuf queue[2], freel[2], shq, bins, spill, film, ctl, stats, L_out, fetch_seg;
+    // path-replay adjoint: per-slot terms, gradients, film adjoint
+    DevBuf a_delta, a_pend_a, a_pend_r, a_nee_da, a_nee_dr, a_gmat, a_glight, a_dfilm;
+    size_t adj_cap = 0;
     size_t spill_stride = 0;
     DevBuf out_a, out_b, out_c;
     KernelTimer timer;

@@ -80,6 +80,19 @@ struct Launch {
     hipStream_t stream;
 };
 
+// Path-replay adjoint state (section "adjoint" of kernels_pt.hip).
+struct AdjBuf {
+    float4* delta;            // dLoss/dLi of the slot's path: film adjoint / spp
+    float4* pend_a;           // BSDF-sample term: (d f / d albedo) / f per channel, w = material bits (-1: none)
+    float4* pend_r;           // (d f / d roughness) / f per channel
+    float4* nee_da;           // beta * d f_nee / d albedo per channel, w = material bits
+    float4* nee_dr;           // beta * d f_nee / d roughness per channel, w = light definition bits
+    const float4* L_fwd;      // radiance of every path id of the round from the forward pass
+    const float* d_film;      // W*H*3 film adjoint, Film.Pixels order
+    float* g_mat;             // 4 per material: albedo rgb, roughness
+    float* g_light;           // 4 per light definition: emission rgb, unused
+};
+
 struct Buffers {
     uint32_t* queue[2];
     uint32_t* freel[2];
@@ -93,6 +106,7 @@ struct Buffers {
     float* film;
     float4* L_out;            // radiance per path id of the round
     uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters (closest, shadow)
+    AdjBuf adj;               // used by Integrator::Adjoint only
 };
 
 // Called around every launch of a round (timing, counters).
@@ -108,13 +122,15 @@ struct LaunchHook {
 // shadow rays for all active paths, whatever their depth.
 //   begin:      counter clear, every slot free
 //   iterations: n (even) x [regen, prep, trace_closest, bin, shade, trace_shadow]
-//   end:        retire the last paths, fold the round into the films
-enum class Integrator { Path, Normal };
+//   end:        retire the last paths, fold the round into the films (not for
+//               adjoint rounds, which only accumulate gradients)
+enum class Integrator { Path, Normal, Adjoint };
 void enqueue_round_begin(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
                          LaunchHook* hook);
 void enqueue_iterations(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
                         int n, bool count, LaunchHook* hook);
-void enqueue_round_end(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, LaunchHook* hook);
+void enqueue_round_end(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                       LaunchHook* hook);
 
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
                 uint32_t* freel, int grid, hipStream_t st);

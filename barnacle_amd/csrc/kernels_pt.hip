@@ -119,9 +119,13 @@ __device__ __forceinline__ uint32_t new_paths(uint32_t n_free, uint32_t next_pid
     return left <= 0 ? 0u : (uint32_t)(left < (int64_t)n_free ? left : (int64_t)n_free);
 }
 
-template <bool BATCH>
+// Adjoint rounds (ADJ) replay the same ids: a new path starts with L = its
+// forward radiance (the radiance still to come, decremented as the replay
+// passes each contribution) and delta = film adjoint / spp; nothing retires.
+template <bool BATCH, bool ADJ>
 __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const Ctl* ctl, int cur,
-                                               const uint32_t* freel_in, uint32_t* queue, float4* L_out) {
+                                               const uint32_t* freel_in, uint32_t* queue, float4* L_out,
+                                               AdjBuf A) {
     const uint32_t n_free = ctl->free_count[cur];
     const uint32_t next_pid = ctl->next_pid;
     const uint32_t n_new = new_paths(n_free, next_pid, W.total_paths());
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const C
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_free; j += gridDim.x * blockDim.x) {
         const uint32_t slot = freel_in[j];
         const uint32_t old = P.pid[slot];
-        if (old != kNoPath) L_out[old] = P.L[slot];
+        if (!ADJ && old != kNoPath) L_out[old] = P.L[slot];
         if (j >= n_new) {
             P.pid[slot] = kNoPath;
             continue;
@@ -138,7 +142,17 @@ __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const C
         const int k = W.item_of_path(id);
         const RoundItem& it = W.item(k);
         int x = 0, y = 0, s = 0;
-        const bool ok = path_pixel(it, id - (uint32_t)it.path_base, x, y, s) && it.max_depth > 0;
+        bool ok = path_pixel(it, id - (uint32_t)it.path_base, x, y, s) && it.max_depth > 0;
+        float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ADJ && ok) {
+            // a path whose pixel has no adjoint (or whose radiance is not
+            // finite) carries no gradient: replay it as a placeholder
+            const float* fp = A.d_film + 3 * ((size_t)(it.height - y - 1) * it.width + x);
+            dl = make_float4(fp[0] * it.inv_spp, fp[1] * it.inv_spp, fp[2] * it.inv_spp, 0.0f);
+            const float4 lf = A.L_fwd[id];
+            ok = (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) && isfinite(lf.x) && isfinite(lf.y) &&
+                 isfinite(lf.z);
+        }
         if (ok) {
             const DevScene& S = W.scene(k);
             uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(it.frame_id * it.spp + s));
@@ -154,7 +168,12 @@ __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const C
             P.o[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
             P.d[slot] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         }
-        P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ADJ) {
+            A.delta[slot] = dl;
+            P.L[slot] = ok ? A.L_fwd[id] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            P.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         P.pid[slot] = id;
         if (BATCH) P.item[slot] = k;
         q[j] = slot;
@@ -511,6 +530,230 @@ __global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, 
     }
 }
 
+// ----------------------------------------------------------------- adjoint
+// Path-replay backpropagation of an image loss to per-material albedo and
+// roughness and per-light emission.  Each path is re-traced from its seed with
+// the forward's decisions; L holds the radiance still to come, decremented as
+// the replay passes each contribution, so no vertex is stored.  Derivatives
+// are taken at fixed sampled directions, light samples and Russian-roulette
+// outcomes, with the importance factors 1/pdf of sampled directions held
+// constant (detached sampling) and the MIS weights differentiated (they
+// depend on roughness through the BSDF pdf): unbiased for the derivative of
+// the expected image, visibility changes excluded.  For a contribution
+// c = beta * e at a later vertex, d c / d theta of the BSDF sample at vertex j
+// is c * (d f_j / d theta) / f_j, so the BSDF term of vertex j is
+// delta * (d f_j / f_j) * L_after(j).  The emission MIS weight of vertex k
+// depends on the pdf sampled at vertex k-1 (material and derivative carried in
+// pend_a.w and beta.w); the NEE weight on the pdf at the light direction.
+//
+// Gradients are summed per wave for lanes sharing a key (material or light)
+// before one atomic per key, and lanes with rare keys add directly.
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ void grad_accum(float* g, int key, float4 v, bool act) {
+    bool pending = act && key >= 0;
+    for (int it = 0; it < 4; ++it) {
+        const uint64_t m = __ballot(pending);
+        if (m == 0) return;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const int k = __shfl(key, leader);
+        const bool mine = pending && key == k;
+        const uint64_t mm = __ballot(mine);
+        if (__popcll(mm) < 4) break;                     // rare keys: direct atomics below
+        const float sx = wave_sum(mine ? v.x : 0.0f), sy = wave_sum(mine ? v.y : 0.0f);
+        const float sz = wave_sum(mine ? v.z : 0.0f), sw = wave_sum(mine ? v.w : 0.0f);
+        if ((int)lane_id() == leader) {
+            float* d = g + 4 * (size_t)k;
+            unsafeAtomicAdd(d + 0, sx); unsafeAtomicAdd(d + 1, sy);
+            unsafeAtomicAdd(d + 2, sz); unsafeAtomicAdd(d + 3, sw);
+        }
+        pending = pending && !mine;
+    }
+    if (pending) {
+        float* d = g + 4 * (size_t)key;
+        unsafeAtomicAdd(d + 0, v.x); unsafeAtomicAdd(d + 1, v.y);
+        unsafeAtomicAdd(d + 2, v.z); unsafeAtomicAdd(d + 3, v.w);
+    }
+}
+__device__ __forceinline__ float ratio(float d, float f) { return f != 0.0f ? d / f : 0.0f; }
+
+// k_shade's bounce, replayed: emission and its emission gradient; NEE set-up
+// with the terms its gradient needs (applied by k_trace_shadow_adj when the
+// shadow ray is unoccluded); the BSDF sample and its term, applied here when
+// no shadow ray follows and after the NEE subtraction otherwise.
+__global__ __launch_bounds__(256) void k_shade_adj(Work<false> W, PathBuf P, AdjBuf A, Ctl* ctl,
+                                                   const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next,
+                                                   int nxt, uint32_t* shq, uint32_t* freel) {
+    uint32_t start[kBins + 1];
+    start[0] = 0;
+    for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
+    const uint32_t total = start[kBins];
+    const DevScene& S = W.scene(0);
+    const RoundItem& it = W.item(0);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false, fin = false;
+        uint32_t slot = 0;
+        int light_key = -1, mat_key = -1, prev_key = -1;
+        float4 g_l = make_float4(0, 0, 0, 0), g_m = make_float4(0, 0, 0, 0), g_p = make_float4(0, 0, 0, 0);
+        if (i < total) {
+            int b = 0;
+            while (i >= start[b + 1]) ++b;
+            slot = bins[(size_t)b * bin_cap + (i - start[b])];
+            const float4 o4 = P.o[slot], d4 = P.d[slot];
+            const Ray r = {xyz(o4), xyz(d4)};
+            const float prev_pdf = d4.w;
+            const int depth = __float_as_int(o4.w);
+            const int4 h = P.hit[slot];
+            const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+            const int4 info = S.insts[h.x].info;
+            V3 L = xyz(P.L[slot]);
+            const float4 b4 = P.beta[slot];
+            V3 beta = xyz(b4);
+            const V3 dlt = xyz(A.delta[slot]);
+            if (info.w >= 0) {
+                V3 Le;
+                float lpdf;
+                light_eval(S, h.x, r.o, g, Le, lpdf);
+                const float w = depth == 0 ? 1.0f : prev_pdf * rcp_e(lpdf + prev_pdf);
+                L = L - beta * (Le * w);
+                const float4 def = S.lightdefs[info.w];
+                const float woz = dot(normalize(r.o - g.p), g.n);
+                const float facing = fabsf(woz) > 1e-6f && (woz > 0.0f || def.w != 0.0f) ? 1.0f : 0.0f;
+                const V3 gl = dlt * beta * (w * facing);
+                g_l = f4(gl, 0.0f);
+                light_key = info.w;
+                if (depth > 0) {
+                    // w = prev / (lpdf + prev): d w / d prev = lpdf / (lpdf + prev)^2
+                    const float sd = lpdf + prev_pdf;
+                    const float dw = lpdf / (sd * sd) * b4.w;
+                    prev_key = __float_as_int(A.pend_a[slot].w);
+                    g_p = make_float4(0.0f, 0.0f, 0.0f, dot(dlt * beta, Le) * dw);
+                }
+            }
+            if (info.z >= 0) {
+                uint32_t rng = P.rng[slot];
+                const Mat m = load_mat(S, info.z);
+                const float dr = dalpha_drough(m);
+                const V3 wo = -r.d;
+                const V3 wo_l = to_local(g, wo);
+                const float uSel = lcg(rng);
+                V2 uL;
+                uL.x = lcg(rng); uL.y = lcg(rng);
+                const LightSample ls = light_sample(S, g.p, uSel, uL);
+                if (ls.pdf != 0.0f) {
+                    const BSDFGrad e = mat_eval_grad(m, wo_l, to_local(g, ls.wi));
+                    const float dist = length(ls.p - g.p);
+                    const float rc = rcp_e(e.pdf + ls.pdf);
+                    P.sh_o[slot] = f4(g.p, dist - 1e-3f);
+                    P.sh_d[slot] = f4(ls.wi, e.dpdf * dr * rc);        // d pdf_b / d rough / (pdf_b + pdf_l)
+                    P.nee_a[slot] = f4(beta * e.f, 0.0f);
+                    P.nee_b[slot] = f4(ls.L * rc, rc * ls.facing);
+                    A.nee_da[slot] = f4(beta * e.dalb, __int_as_float(info.z));
+                    A.nee_dr[slot] = f4(beta * e.dalpha * dr, __int_as_float(ls.ldef));
+                    shadow = true;
+                }
+                const float ul = lcg(rng);
+                V2 ub;
+                ub.x = lcg(rng); ub.y = lcg(rng);
+                V3 wi_l;
+                const BSDFGrad bs = mat_sample_grad(m, wo_l, ul, ub, wi_l);
+                V3 pa = mk(0, 0, 0), pr = mk(0, 0, 0);
+                if (bs.pdf != 0.0f) {
+                    const V3 wi = to_world(g, wi_l);
+                    beta = beta * bs.f * rcp_e(bs.pdf);
+                    cont = true;
+                    if (depth >= it.rr_depth) {
+                        const float q = min_n(1.0f, max_n(beta.x, max_n(beta.y, beta.z)));
+                        if (lcg(rng) < q) beta = beta * rcp_e(q);
+                        else cont = false;
+                    }
+                    P.o[slot] = f4(g.p, __int_as_float(depth + 1));
+                    P.d[slot] = f4(wi, bs.pdf);
+                    P.beta[slot] = f4(beta, bs.dpdf * dr);                // d prev pdf / d rough
+                    pa = {ratio(bs.dalb.x, bs.f.x), ratio(bs.dalb.y, bs.f.y), ratio(bs.dalb.z, bs.f.z)};
+                    pr = {ratio(bs.dalpha.x * dr, bs.f.x), ratio(bs.dalpha.y * dr, bs.f.y),
+                          ratio(bs.dalpha.z * dr, bs.f.z)};
+                }
+                P.rng[slot] = rng;
+                cont = cont && depth + 1 < it.max_depth;
+                const int pend = cont ? info.z : -1;           // no later radiance: no term
+                A.pend_a[slot] = f4(pa, __int_as_float(pend));   // w: also the next vertex's prev material
+                A.pend_r[slot] = f4(pr, 0.0f);
+                if (!shadow && pend >= 0) {
+                    const V3 t = dlt * pa * L;
+                    g_m = make_float4(t.x, t.y, t.z, dot(dlt * pr, L));
+                    mat_key = pend;
+                }
+            }
+            P.L[slot] = f4(L, 0.0f);
+            fin = !cont;
+        }
+        grad_accum(A.g_light, light_key, g_l, light_key >= 0);
+        grad_accum(A.g_mat, prev_key, g_p, prev_key >= 0);
+        grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
+        block_push<3>({cont, shadow, fin}, {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
+                      {queue_next, shq, freel}, slot);
+    }
+}
+
+// The shadow ray of a replayed bounce: when unoccluded, the NEE contribution
+// leaves the radiance still to come and adds its gradients (the vertex's
+// material through f, the sampled light through its emission); then the
+// vertex's BSDF-sample term is applied.
+template <int K>
+__global__ __launch_bounds__(256) void k_trace_shadow_adj(Work<false> W, PathBuf P, AdjBuf A, Ctl* ctl,
+                                                          const uint32_t* shq, uint32_t* spill,
+                                                          size_t spill_stride, uint32_t refill, uint32_t* seg) {
+    __shared__ uint32_t stk[K * 256];
+    const uint32_t count = ctl->sh_count;
+    TraceCount tc{0, 0, 0};
+    persistent_trace<true, false, K>(
+        SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<K>& tr, uint32_t i) {
+            const uint32_t slot = shq[i];
+            const float4 o4 = P.sh_o[slot];
+            tr.start(W.scene(0), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
+            if (!(o4.w == o4.w)) tr.st.sp = 0;
+        },
+        [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
+            int mat_key = -1, light_key = -1;
+            float4 g_m = make_float4(0, 0, 0, 0), g_l = make_float4(0, 0, 0, 0);
+            if (fin) {
+                const uint32_t slot = shq[i];
+                V3 L = xyz(P.L[slot]);
+                const V3 dlt = xyz(A.delta[slot]);
+                V3 gm = mk(0, 0, 0);
+                float gr = 0.0f;
+                if (!found) {
+                    const float4 a = P.nee_a[slot], b = P.nee_b[slot];
+                    const float4 da = A.nee_da[slot], dr = A.nee_dr[slot];
+                    const V3 c = xyz(a) * xyz(b);
+                    L = L - c;
+                    gm = dlt * xyz(da) * xyz(b);
+                    // d (f Le / (pdf_b + pdf_l)) / d rough = f' Le / S - c pdf_b' / S
+                    gr = dot(dlt, xyz(dr) * xyz(b) - c * P.sh_d[slot].w);
+                    mat_key = __float_as_int(da.w);
+                    light_key = __float_as_int(dr.w);
+                    g_l = f4(dlt * xyz(a) * b.w, 0.0f);
+                }
+                const float4 pa = A.pend_a[slot];
+                const int pend = __float_as_int(pa.w);
+                if (pend >= 0) {
+                    gm = gm + dlt * xyz(pa) * L;
+                    gr += dot(dlt * xyz(A.pend_r[slot]), L);
+                    mat_key = pend;
+                }
+                g_m = f4(gm, gr);
+                P.L[slot] = f4(L, 0.0f);
+            }
+            grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
+            grad_accum(A.g_light, light_key, g_l, light_key >= 0);
+        });
+}
+
 // -------------------------------------------------------------------- film
 // accum = FMA(1/spp, Li / pdf, accum) in sample order (Integrator.fs:41-42);
 // the camera pdf is 1.  The film keeps Film.Pixels order (Film.fs:43).  One
@@ -656,8 +899,12 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
     for (int i = 0; i < n; ++i) {
         const int cur = i & 1, nxt = 1 - cur;
         { Bracket b(hook, "regen", st);
-          hipLaunchKernelGGL(k_regen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, cur, B.freel[cur],
-                             B.queue[cur], B.L_out); }
+          if (kind == Integrator::Adjoint)
+              hipLaunchKernelGGL((k_regen<BATCH, true>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, cur,
+                                 B.freel[cur], B.queue[cur], B.L_out, B.adj);
+          else
+              hipLaunchKernelGGL((k_regen<BATCH, false>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, cur,
+                                 B.freel[cur], B.queue[cur], B.L_out, B.adj); }
         { Bracket b(hook, "prep", st);
           hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg); }
         { Bracket b(hook, "trace_closest", st);
@@ -671,6 +918,20 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
                 hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
                                    nxt, B.freel[nxt]);
             }
+        } else if (kind == Integrator::Adjoint) {
+            if constexpr (!BATCH) {
+                { Bracket b(hook, "shade_adj", st);
+                  hipLaunchKernelGGL(k_shade_adj, dim3(L.grid_shade), dim3(256), 0, st, W, P, B.adj, ctl, B.bins,
+                                     B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
+                { Bracket b(hook, "trace_shadow_adj", st);
+                  uint32_t* seg = B.fetch_seg + kFetchSegs * kSegStride;
+                  if (B.stack_k == 16)
+                      hipLaunchKernelGGL(k_trace_shadow_adj<16>, dim3(L.grid_trace), dim3(256), 0, st, W, P, B.adj,
+                                         ctl, B.shq, B.spill, B.spill_stride, L.refill, seg);
+                  else
+                      hipLaunchKernelGGL(k_trace_shadow_adj<32>, dim3(L.grid_trace), dim3(256), 0, st, W, P, B.adj,
+                                         ctl, B.shq, B.spill, B.spill_stride, L.refill, seg); }
+            }
         } else {
             { Bracket b(hook, "shade", st);
               hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
@@ -683,13 +944,14 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
 }
 
 template <bool BATCH>
-static void round_end_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
-                        LaunchHook* hook) {
+static void round_end_t(const Work<BATCH>& W, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl,
+                        const Buffers& B, LaunchHook* hook) {
     hipStream_t st = L.stream;
     // iteration blocks are even: the last paths sit in free list 0
+    if (kind == Integrator::Adjoint) return;               // nothing to retire, no film
     { Bracket b(hook, "regen", st);
-      hipLaunchKernelGGL(k_regen<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, 0, B.freel[0],
-                         B.queue[0], B.L_out); }
+      hipLaunchKernelGGL((k_regen<BATCH, false>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, 0, B.freel[0],
+                         B.queue[0], B.L_out, B.adj); }
     { Bracket b(hook, "film", st);
       hipLaunchKernelGGL(k_film<BATCH>, dim3(BATCH ? L.grid_shade : (L.n_pix + 255) / 256), dim3(256), 0, st, W,
                          B.L_out, B.film); }
@@ -705,9 +967,10 @@ void enqueue_iterations(bool batch, Integrator kind, const Launch& L, const Path
     if (batch) iterations_t<true>(batch_work(L), kind, L, P, ctl, B, n, count, hook);
     else iterations_t<false>(plain_work(L), kind, L, P, ctl, B, n, count, hook);
 }
-void enqueue_round_end(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B, LaunchHook* hook) {
-    if (batch) round_end_t<true>(batch_work(L), L, P, ctl, B, hook);
-    else round_end_t<false>(plain_work(L), L, P, ctl, B, hook);
+void enqueue_round_end(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
+                       LaunchHook* hook) {
+    if (batch) round_end_t<true>(batch_work(L), kind, L, P, ctl, B, hook);
+    else round_end_t<false>(plain_work(L), kind, L, P, ctl, B, hook);
 }
 
 // Single kernels used by the PSSMLT driver (plain scene).

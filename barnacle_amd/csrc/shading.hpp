@@ -48,10 +48,10 @@ __device__ __forceinline__ V3 to_local(const Geom& g, V3 v) { return {dot(v, g.t
 __device__ __forceinline__ V3 to_world(const Geom& g, V3 v) { return v.x * g.t + v.y * g.b + v.z * g.n; }
 
 // ------------------------------------------------------------- materials
-struct Mat { int kind; V3 albedo; float ior, alpha, metallic; };
+struct Mat { int kind; V3 albedo; float ior, alpha, metallic, roughness; };
 __device__ __forceinline__ Mat load_mat(const DevScene& S, int m) {
     float4 a = S.mats[2 * m], b = S.mats[2 * m + 1];
-    return {__float_as_int(a.x), {a.y, a.z, a.w}, b.x, b.y, b.z};
+    return {__float_as_int(a.x), {a.y, a.z, a.w}, b.x, b.y, b.z, b.w};
 }
 struct BSDF { V3 f; float pdf; };
 
@@ -148,7 +148,81 @@ __device__ __forceinline__ V3 light_emit(float4 def, float woz) {        // Ligh
     if (fabsf(woz) > 1e-6f && (woz > 0.0f || def.w != 0.0f)) return xyz(def);
     return {0, 0, 0};
 }
-struct LightSample { V3 p, L; float pdf; V3 wi; };
+struct LightSample { V3 p, L; float pdf; V3 wi; int ldef; float facing; };
+
+// ------------------------------------------------------------ derivatives
+// d f / d albedo (per channel; f_c depends on albedo_c only) and d f / d alpha
+// (per channel) of a BSDF value at fixed directions, for the path-replay
+// adjoint.  The value and pdf are those of mat_eval / mat_sample, bit for bit.
+struct BSDFGrad { V3 f; float pdf; V3 dalb; V3 dalpha; float dpdf; };   // dpdf: d pdf / d alpha
+
+__device__ __forceinline__ float pbr_dlambda(float alpha, V3 w) {
+    float s2 = fmaf(w.x, w.x, w.y * w.y);
+    if (s2 == 0.0f) return 0.0f;
+    float t2 = s2 / (w.z * w.z);
+    return 0.5f * alpha * t2 / sqrtf(1.0f + alpha * alpha * t2);
+}
+__device__ __forceinline__ BSDFGrad pbr_eval_grad(const Mat& m, V3 wo, V3 wi) {
+    const BSDF e = pbr_eval(m, wo, wi);
+    V3 wh = normalize(wo + wi);
+    float c = fabsf(wh.z);
+    float a = m.alpha;
+    float x = 1.0f + (a * a - 1.0f) * c * c;
+    float d = a * a / (kPi * (x * x));
+    float dd = 2.0f * a / (kPi * x * x) * (1.0f - 2.0f * a * a * c * c / x);
+    float g = 1.0f / (1.0f + pbr_lambda(a, wo) + pbr_lambda(a, wi));
+    float dg = -g * g * (pbr_dlambda(a, wo) + pbr_dlambda(a, wi));
+    float spec = d * g / (4.0f * fabsf(wo.z));
+    float dspec = (dd * g + d * dg) / (4.0f * fabsf(wo.z));
+    float k = 1.0f - dot(wo, wh);
+    float k5 = k * k * k * k * k;
+    float F = 0.04f + 0.96f * k5;
+    float sm = 1.0f - m.metallic;
+    float base = sm * (1.0f - F) * fabsf(wi.z) / kPi + m.metallic * spec * (1.0f - k5);
+    V3 fc = m.albedo + (mk(1, 1, 1) - m.albedo) * k5;
+    BSDFGrad r;
+    r.f = e.f;
+    r.pdf = e.pdf;
+    r.dalb = mk(base, base, base);
+    r.dalpha = dspec * (mk(sm * F, sm * F, sm * F) + m.metallic * fc);
+    r.dpdf = dd * c / (4.0f * dot(wo, wh)) * (1.0f - 0.5f * (1.0f - m.metallic));
+    return r;
+}
+__device__ __forceinline__ BSDFGrad mat_eval_grad(const Mat& m, V3 wo, V3 wi) {
+    if (m.kind == 3) return pbr_eval_grad(m, wo, wi);
+    const BSDF e = mat_eval(m, wo, wi);
+    BSDFGrad r;
+    r.f = e.f;
+    r.pdf = e.pdf;
+    r.dalb = m.kind == 0 && e.pdf != 0.0f ? mk(e.pdf, e.pdf, e.pdf) : mk(0, 0, 0);
+    r.dalpha = mk(0, 0, 0);
+    r.dpdf = 0.0f;
+    return r;
+}
+__device__ __forceinline__ BSDFGrad mat_sample_grad(const Mat& m, V3 wo, float ul, V2 u, V3& wi) {
+    const BSDF e = mat_sample(m, wo, ul, u, wi);
+    if (m.kind == 3) {
+        BSDFGrad r = pbr_eval_grad(m, wo, wi);
+        r.f = e.f;
+        r.pdf = e.pdf;
+        return r;
+    }
+    BSDFGrad r;
+    r.f = e.f;
+    r.pdf = e.pdf;
+    // Lambertian: albedo * pdf; mirror / total reflection: albedo; dielectric:
+    // r * albedo or (1 - r) * albedo with the returned pdf r or 1 - r.
+    const float k = m.kind == 0 ? e.pdf : (m.kind == 1 ? 1.0f : e.pdf);
+    r.dalb = mk(k, k, k);
+    r.dalpha = mk(0, 0, 0);
+    r.dpdf = 0.0f;
+    return r;
+}
+// d alpha / d roughness of Alpha = Max(roughness^2, 1e-3) (PBR.fs:86)
+__device__ __forceinline__ float dalpha_drough(const Mat& m) {
+    float r2 = m.roughness * m.roughness;
+    return r2 > 1e-3f ? 2.0f * m.roughness : 0.0f;
+}
 
 // UniformLightSampler.Sample (Uniform.fs:13-29) over MeshInstance.Sample
 // (Mesh.fs:289-298, AliasTable.fs:52-62, Triangle.Sample Mesh.fs:89-99) or
@@ -202,9 +276,13 @@ __device__ __forceinline__ LightSample light_sample(const DevScene& S, V3 p, flo
     float dist2 = dot(dv, dv);
     LightSample s;
     s.p = lp;
-    s.L = light_emit(S.lightdefs[rec.w], dot(wo, ln));
+    const float4 def = S.lightdefs[rec.w];
+    const float woz = dot(wo, ln);
+    s.L = light_emit(def, woz);
     s.pdf = dist2 * pdfS / (max_n(fabsf(cosWo), 1e-6f) * (float)N);
     s.wi = -wo;
+    s.ldef = rec.w;
+    s.facing = fabsf(woz) > 1e-6f && (woz > 0.0f || def.w != 0.0f) ? 1.0f : 0.0f;
     return s;
 }
 

@@ -59,6 +59,8 @@ _SIGS = {
                          C.POINTER(abi.BatchStats)], C.c_int),
     "bn_autotune": ([C.c_void_p, C.POINTER(abi.BatchItem), C.c_int32, C.c_int32, C.POINTER(abi.TuneResult)],
                     C.c_int),
+    "bn_set_params": ([C.c_void_p, C.c_int32, f32p, C.c_int32, f32p, C.c_int32], C.c_int),
+    "bn_backward_pt": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p, f32p, f32p], C.c_int),
     "bn_pssmlt_last": ([C.c_void_p, f32p, C.c_int32, i32p, C.c_int32], C.c_int),
 }
 
@@ -210,6 +212,25 @@ class DeviceScene:
         check(self.ctx.lib.bn_pssmlt_last(self.ctx.h, _p(boot, f32p), params.n_bootstrap, _p(acc, i32p),
                                           params.n_chains))
         return boot, acc
+
+    def set_params(self, mat_params=None, light_emission=None):
+        """Albedo rgb + roughness per material ((n_mat, 4)) and emission per
+        light ((n_light, 3)); None keeps the current values."""
+        mp = None if mat_params is None else np.ascontiguousarray(mat_params, np.float32).reshape(-1, 4)
+        le = None if light_emission is None else np.ascontiguousarray(light_emission, np.float32).reshape(-1, 3)
+        check(self.ctx.lib.bn_set_params(self.ctx.h, self.id, None if mp is None else _p(mp, f32p),
+                                         0 if mp is None else len(mp), None if le is None else _p(le, f32p),
+                                         0 if le is None else len(le)))
+
+    def backward_pt(self, params: abi.PTParams, d_film):
+        """Gradients of sum(d_film * film): ((n_mat, 4), (n_light, 3))."""
+        df = np.ascontiguousarray(d_film, np.float32).reshape(-1, 3)
+        assert len(df) == params.width * params.height
+        dm = np.zeros((max(1, len(self.sc.materials)), 4), np.float32)
+        dl = np.zeros((max(1, len(self.sc.lights)), 3), np.float32)
+        check(self.ctx.lib.bn_backward_pt(self.ctx.h, self.id, C.byref(params), _p(df, f32p), _p(dm, f32p),
+                                          _p(dl, f32p)))
+        return dm[:len(self.sc.materials)], dl[:len(self.sc.lights)]
 
     def export(self):
         cap = 4 * max(1, len(self.sc.instances)) * 64 + 16

@@ -323,6 +323,27 @@ typedef struct bn_tune_result {
 } bn_tune_result;
 int bn_autotune(bn_ctx* ctx, const bn_batch_item* items, int32_t n, int32_t flags, bn_tune_result* out);
 
+/* Differentiable rendering (no reference counterpart; the adjoint of
+ * PathTracingIntegrator.Li, PathTracing.fs:14-81).  bn_set_params replaces the
+ * albedo and roughness of every material (4 floats each: albedo rgb,
+ * roughness; MaterialInfo, Loader.fs:50-73) and the emission of every light
+ * (3 floats each; LightInfo, Loader.fs:75-90) of a scene; either array may be
+ * NULL to keep it, and its count must equal the scene's.  bn_render_pt is the
+ * forward pass.  bn_backward_pt returns the gradient of sum(d_film * film)
+ * for the frame bn_render_pt renders with the same parameters, by path-replay
+ * backpropagation: every path is re-traced from its seed, nothing per vertex
+ * is stored.  Derivatives are taken at fixed sampled directions, light
+ * samples and Russian-roulette outcomes, with the importance factors 1/pdf
+ * held constant and the MIS weights differentiated (unbiased for the
+ * derivative of the expected image, visibility changes excluded).  Pixels
+ * whose d_film is zero cost nothing; a path whose radiance is not finite
+ * contributes nothing.  d_mat: 4 per material (d albedo rgb, d roughness);
+ * d_light: 3 per light (d emission rgb); either may be NULL. */
+int bn_set_params(bn_ctx* ctx, int32_t scene, const float* mat_params, int32_t n_mat,
+                  const float* light_emission, int32_t n_light);
+int bn_backward_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, const float* d_film,
+                   float* d_mat, float* d_light);
+
 /* PSSMLTIntegrator.Render (Extensions/Integrator/PSSMLT.fs:379-414).  The film
  * is cleared first (Film.Clear, Render.fs:11) and accumulated with device
  * atomics (the reference's Film.Accumulate is an unsynchronised add). */

@@ -370,7 +370,7 @@ struct Inst {
     AABB bounds;
 };
 
-struct Material { int kind; V3 albedo; float ior, alpha, metallic; };
+struct Material { int kind; V3 albedo; float ior, alpha, metallic, roughness; };
 struct LightD { V3 emission; bool twoSided; };
 
 struct Geom { V3 p; ONB onb; };
@@ -710,6 +710,7 @@ Material toMaterial(const orc_material& d) {
     m.albedo = {d.albedo[0], d.albedo[1], d.albedo[2]};
     m.ior = d.ior;
     m.alpha = maxN(d.roughness * d.roughness, 1e-3f);      // PBR.fs:86
+    m.roughness = d.roughness;
     float mt = d.metallic;                                  // Single.Clamp (PBR.fs:87)
     m.metallic = mt < 0.0f ? 0.0f : (mt > 1.0f ? 1.0f : mt);
     return m;
@@ -720,7 +721,7 @@ inline V3 lightEmit(const LightD& l, float woz) {         // Light.fs:49-52
     if (std::fabs(woz) > 1e-6f && (woz > 0.0f || l.twoSided)) return l.emission;
     return {0, 0, 0};
 }
-struct LightSample { V3 p, L; float pdf; V3 wi; };
+struct LightSample { V3 p, L; float pdf; V3 wi; int ldef; float facing; };
 // MeshInstance.Sample (Mesh.fs:289-298) / SphereInstance.Sample (Sphere.fs:90-113)
 float instSample(const orc_scene& s, const Inst& in, float uSel, V2 u, V3& p, V3& n) {
     const Mesh& m = s.meshes[in.prim];
@@ -767,9 +768,13 @@ LightSample lightSample(const orc_scene& s, V3 p, float uSel, V2 u) {
     float dist2 = dot(dv, dv);
     LightSample r;
     r.p = lp;
-    r.L = lightEmit(s.lightDefs[in.light], dot(wo, ln));
+    const LightD& ld = s.lightDefs[in.light];
+    const float woz = dot(wo, ln);
+    r.L = lightEmit(ld, woz);
     r.pdf = dist2 * pdfS / (maxN(std::fabs(cosWo), 1e-6f) * (float)N);
     r.wi = -wo;
+    r.ldef = in.light;
+    r.facing = std::fabs(woz) > 1e-6f && (woz > 0.0f || ld.twoSided) ? 1.0f : 0.0f;
     return r;
 }
 // PrimitiveInstance.EvalPDF (Mesh.fs:300-304 with the tag lost; Sphere.fs:115-126)
@@ -1243,6 +1248,214 @@ static void fillStats(const Stats& a, orc_stats* o) {
     o->vertices = a.verts; o->paths = a.paths;
 }
 
+// ------------------------------------------------------------ adjoint
+// FP64 restatement of the derivative the HIP path-replay adjoint computes:
+// every path is traced exactly as liPT does (float32, same decisions), its
+// vertices are kept, and the derivative of each radiance contribution with
+// respect to material albedo / roughness and light emission is accumulated in
+// double by direct differentiation (product rule over the BSDF factors of the
+// earlier vertices).  Sampled directions, light samples, Russian-roulette
+// outcomes and the importance factors 1/pdf are held fixed; the MIS weights
+// are differentiated (they depend on roughness through the BSDF pdf).
+
+// PBRMaterial.Eval value (PBR.fs:112-124) in double.
+void pbrFD(const double alb[3], double alpha, double met, const double wo[3], const double wi[3], double f[3],
+           double* pdf = nullptr) {
+    double h[3] = {wo[0] + wi[0], wo[1] + wi[1], wo[2] + wi[2]};
+    double hl = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    for (double& v : h) v /= hl;
+    double c = std::fabs(h[2]);
+    double x = 1.0 + (alpha * alpha - 1.0) * c * c;
+    double d = alpha * alpha / (M_PI * x * x);
+    auto lam = [&](const double w[3]) {
+        double s2 = w[0] * w[0] + w[1] * w[1];
+        if (s2 == 0.0) return 0.0;
+        double t2 = s2 / (w[2] * w[2]);
+        return (-1.0 + std::sqrt(1.0 + alpha * alpha * t2)) / 2.0;
+    };
+    double g = 1.0 / (1.0 + lam(wo) + lam(wi));
+    double spec = d * g / (4.0 * std::fabs(wo[2]));
+    double k = 1.0 - (wo[0] * h[0] + wo[1] * h[1] + wo[2] * h[2]);
+    double k5 = k * k * k * k * k;
+    double F = 0.04 + 0.96 * k5;
+    for (int ch = 0; ch < 3; ++ch) {
+        double diff = alb[ch] * std::fabs(wi[2]) / M_PI;
+        double fc = alb[ch] + (1.0 - alb[ch]) * k5;
+        double inner = diff * (1.0 - F) + spec * F;
+        f[ch] = inner * (1.0 - met) + spec * fc * met;
+    }
+    if (pdf) {
+        double cw = wo[0] * h[0] + wo[1] * h[1] + wo[2] * h[2];
+        double tt = 0.5 * (1.0 - met);
+        *pdf = d * c / (4.0 * cw) * (1.0 - tt) + std::fabs(wi[2]) / M_PI * tt;
+    }
+}
+// Its derivatives: d f_c / d albedo_c and d f_c / d alpha.
+void pbrGradD(const double alb[3], double alpha, double met, const double wo[3], const double wi[3], double da[3],
+              double dal[3], double* dpdf = nullptr) {
+    double h[3] = {wo[0] + wi[0], wo[1] + wi[1], wo[2] + wi[2]};
+    double hl = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    for (double& v : h) v /= hl;
+    double c = std::fabs(h[2]);
+    double a2 = alpha * alpha;
+    double x = 1.0 + (a2 - 1.0) * c * c;
+    double d = a2 / (M_PI * x * x);
+    double dd = 2.0 * alpha / (M_PI * x * x) - 4.0 * alpha * a2 * c * c / (M_PI * x * x * x);
+    auto lam = [&](const double w[3], double& dl) {
+        double s2 = w[0] * w[0] + w[1] * w[1];
+        dl = 0.0;
+        if (s2 == 0.0) return 0.0;
+        double t2 = s2 / (w[2] * w[2]);
+        double r = std::sqrt(1.0 + a2 * t2);
+        dl = alpha * t2 / (2.0 * r);
+        return (-1.0 + r) / 2.0;
+    };
+    double dlo, dli;
+    double lo = lam(wo, dlo), li = lam(wi, dli);
+    double g = 1.0 / (1.0 + lo + li);
+    double dg = -g * g * (dlo + dli);
+    double spec = d * g / (4.0 * std::fabs(wo[2]));
+    double dspec = (dd * g + d * dg) / (4.0 * std::fabs(wo[2]));
+    double k = 1.0 - (wo[0] * h[0] + wo[1] * h[1] + wo[2] * h[2]);
+    double k5 = k * k * k * k * k;
+    double F = 0.04 + 0.96 * k5;
+    for (int ch = 0; ch < 3; ++ch) {
+        double fc = alb[ch] + (1.0 - alb[ch]) * k5;
+        da[ch] = (1.0 - met) * (1.0 - F) * std::fabs(wi[2]) / M_PI + met * spec * (1.0 - k5);
+        dal[ch] = dspec * ((1.0 - met) * F + met * fc);
+    }
+    if (dpdf) {
+        double cw = wo[0] * h[0] + wo[1] * h[1] + wo[2] * h[2];
+        *dpdf = dd * c / (4.0 * cw) * (1.0 - 0.5 * (1.0 - met));
+    }
+}
+// d f / d albedo and d f / d roughness at fixed directions for every kind;
+// f_pdf is the pdf the forward returned with the value (Lambertian albedo*pdf,
+// mirror albedo*1, dielectric albedo*{r, 1-r, 1}).
+// dpdf: d pdf / d roughness (the BSDF pdf enters the MIS weights).
+void matGradD(const Material& m, V3 wo, V3 wi, float f_pdf, double da[3], double dr[3], double* dpdf = nullptr) {
+    for (int ch = 0; ch < 3; ++ch) { da[ch] = 0.0; dr[ch] = 0.0; }
+    if (dpdf) *dpdf = 0.0;
+    if (m.kind == 3) {
+        double alb[3] = {m.albedo.x, m.albedo.y, m.albedo.z};
+        double o[3] = {wo.x, wo.y, wo.z}, i[3] = {wi.x, wi.y, wi.z};
+        double dal[3], dp;
+        pbrGradD(alb, m.alpha, m.metallic, o, i, da, dal, &dp);
+        float r = m.roughness;
+        double dadr = r * r > 1e-3f ? 2.0 * r : 0.0;
+        for (int ch = 0; ch < 3; ++ch) dr[ch] = dal[ch] * dadr;
+        if (dpdf) *dpdf = dp * dadr;
+        return;
+    }
+    for (int ch = 0; ch < 3; ++ch) da[ch] = f_pdf;
+}
+
+struct GradFactor { int mat; double la[3], lr[3]; };
+
+void liPTGrad(const orc_scene& s, Ray ray, Sampler& smp, int maxDepth, int rrDepth, const double delta[3],
+              double* gm, double* gl) {
+    if (delta[0] == 0.0 && delta[1] == 0.0 && delta[2] == 0.0) return;   // no adjoint: no gradient
+    V3 beta = {1, 1, 1};
+    int depth = 0;
+    float prevPdf = 0.0f;
+    int prevMat = -1;                                       // material whose pdf is prevPdf
+    double dPrevPdf = 0.0;                                  // d prevPdf / d roughness of prevMat
+    std::vector<GradFactor> fac;
+    Stats st;
+    // one contribution c (rgb) reaching the film through beta: the earlier
+    // BSDF factors' terms
+    auto through = [&](const double c[3]) {
+        for (const GradFactor& f : fac) {
+            double* g = gm + 4 * (size_t)f.mat;
+            for (int ch = 0; ch < 3; ++ch) {
+                g[ch] += delta[ch] * c[ch] * f.la[ch];
+                g[3] += delta[ch] * c[ch] * f.lr[ch];
+            }
+        }
+    };
+    while (depth < maxDepth) {
+        float t = INF_F;
+        Hit h;
+        if (!closest(s, ray, h, t, st.cc)) break;
+        const Inst& in = s.insts[h.inst];
+        if (in.light >= 0) {
+            LightEval le = lightEval(s, ray.o, h);
+            float w = depth == 0 ? 1.0f : prevPdf * rcpE(le.pdf + prevPdf);
+            V3 lw = le.L * w;
+            double c[3] = {(double)beta.x * lw.x, (double)beta.y * lw.y, (double)beta.z * lw.z};
+            through(c);
+            float woz = toLocal(h.g.onb, normalize(ray.o - h.g.p)).z;
+            const LightD& ld = s.lightDefs[in.light];
+            double facing = std::fabs(woz) > 1e-6f && (woz > 0.0f || ld.twoSided) ? 1.0 : 0.0;
+            double bw[3] = {beta.x, beta.y, beta.z};
+            for (int ch = 0; ch < 3; ++ch) gl[3 * (size_t)in.light + ch] += delta[ch] * bw[ch] * w * facing;
+            if (depth > 0 && prevMat >= 0) {
+                // w = prev / (lpdf + prev): d w / d prev = lpdf / (lpdf + prev)^2
+                double sden = (double)le.pdf + prevPdf;
+                double dw = (double)le.pdf / (sden * sden) * dPrevPdf;
+                double lv[3] = {le.L.x, le.L.y, le.L.z};
+                for (int ch = 0; ch < 3; ++ch) gm[4 * (size_t)prevMat + 3] += delta[ch] * bw[ch] * lv[ch] * dw;
+            }
+        }
+        if (in.material < 0) break;
+        const Material& m = s.mats[in.material];
+        float uSel = smp.next();
+        V2 uL = smp.next2();
+        LightSample ls = lightSample(s, h.g.p, uSel, uL);
+        Ray sh = {h.g.p, ls.wi};
+        float dist = length(ls.p - h.g.p);
+        if (ls.pdf != 0.0f && !anyHit(s, sh, dist - 1e-3f, st.cs)) {
+            BSDFEval e;
+            evalBSDF(s, h, -ray.d, ls.wi, e);
+            float rc = rcpE(e.pdf + ls.pdf);
+            V3 lb = ls.L * rc;
+            V3 a = beta * e.bsdf;
+            double c[3] = {(double)a.x * lb.x, (double)a.y * lb.y, (double)a.z * lb.z};
+            through(c);
+            double da[3], dr[3], dpb;
+            matGradD(m, toLocal(h.g.onb, -ray.d), toLocal(h.g.onb, ls.wi), e.pdf, da, dr, &dpb);
+            double* g = gm + 4 * (size_t)in.material;
+            double bl[3] = {(double)beta.x * lb.x, (double)beta.y * lb.y, (double)beta.z * lb.z};
+            // d / d roughness of f / (pdf_b + pdf_l): f' / S - f * pdf_b' / S^2
+            double dmis = dpb * (double)rc;
+            for (int ch = 0; ch < 3; ++ch) {
+                g[ch] += delta[ch] * bl[ch] * da[ch];
+                g[3] += delta[ch] * bl[ch] * dr[ch] - delta[ch] * c[ch] * dmis;
+            }
+            // the sampled light's emission (Light.fs:49-52 facing test)
+            const double av[3] = {a.x, a.y, a.z};
+            for (int ch = 0; ch < 3; ++ch)
+                gl[3 * (size_t)ls.ldef + ch] += delta[ch] * av[ch] * rc * ls.facing;
+        }
+        float ul = smp.next();
+        V2 ub = smp.next2();
+        BSDFSample bs = sampleBSDF(s, h, -ray.d, ul, ub);
+        prevPdf = bs.e.pdf;
+        if (bs.e.pdf == 0.0f) break;
+        {
+            GradFactor f;
+            f.mat = in.material;
+            double da[3], dr[3];
+            matGradD(m, toLocal(h.g.onb, -ray.d), toLocal(h.g.onb, bs.wi), bs.e.pdf, da, dr, &dPrevPdf);
+            prevMat = in.material;
+            const float fv[3] = {bs.e.bsdf.x, bs.e.bsdf.y, bs.e.bsdf.z};
+            for (int ch = 0; ch < 3; ++ch) {
+                f.la[ch] = fv[ch] != 0.0f ? da[ch] / fv[ch] : 0.0;
+                f.lr[ch] = fv[ch] != 0.0f ? dr[ch] / fv[ch] : 0.0;
+            }
+            fac.push_back(f);
+        }
+        ray = {h.g.p, bs.wi};
+        beta = beta * bs.e.bsdf * rcpE(bs.e.pdf);
+        if (depth >= rrDepth) {
+            float q = minN(1.0f, maxN(beta.x, maxN(beta.y, beta.z)));
+            if (smp.next() < q) beta = beta * rcpE(q);
+            else break;
+        }
+        depth = depth + 1;
+    }
+}
+
 int orc_render(orc_scene* s, int32_t integ, const orc_pt_params* p, int32_t n_threads, float* film,
                orc_stats* stats, int32_t x0, int32_t y0, int32_t x1, int32_t y1) {
     if (s->lights.empty() && integ != 0) return fail("LightSamplerBase: No light primitives found.");
@@ -1284,6 +1497,66 @@ int orc_trace_sample(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y,
     Ray r = generatePrimary(*s, p->width, p->height, x, y, uPix, uLens);
     V3 L = liPT(*s, r, smp, p->max_depth, p->rr_depth, st);
     rgb[0] = L.x; rgb[1] = L.y; rgb[2] = L.z;
+    return 0;
+}
+
+int orc_set_params(orc_scene* s, const float* mp, int32_t n_mat, const float* le, int32_t n_light) {
+    if (mp) {
+        if (n_mat != (int)s->mats.size()) return fail("material parameter count mismatch");
+        for (int i = 0; i < n_mat; ++i) {
+            Material& m = s->mats[i];
+            m.albedo = {mp[4 * i], mp[4 * i + 1], mp[4 * i + 2]};
+            m.roughness = mp[4 * i + 3];
+            m.alpha = maxN(m.roughness * m.roughness, 1e-3f);
+        }
+    }
+    if (le) {
+        if (n_light != (int)s->lightDefs.size()) return fail("light parameter count mismatch");
+        for (int i = 0; i < n_light; ++i) s->lightDefs[i].emission = {le[3 * i], le[3 * i + 1], le[3 * i + 2]};
+    }
+    return 0;
+}
+
+int orc_adjoint_pt(orc_scene* s, const orc_pt_params* p, const float* d_film, int32_t n_threads, int32_t x0,
+                   int32_t y0, int32_t x1, int32_t y1, double* d_mat, double* d_light) {
+    if (s->lights.empty()) return fail("LightSamplerBase: No light primitives found.");
+    if (x1 <= x0 || y1 <= y0) { x0 = 0; y0 = 0; x1 = p->width; y1 = p->height; }
+    const size_t nm = 4 * s->mats.size(), nl = 3 * s->lightDefs.size();
+    int nt = std::max(1, (int)n_threads);
+    std::vector<std::vector<double>> gm(nt, std::vector<double>(nm, 0.0)), gl(nt, std::vector<double>(nl, 0.0));
+    std::atomic<int> next(0);
+    const float inv_spp = 1.0f / (float)p->spp;
+    auto work = [&](int w) {
+        for (;;) {
+            int y = y0 + next.fetch_add(1);
+            if (y >= y1) break;
+            for (int x = x0; x < x1; ++x) {
+                size_t idx = (size_t)(p->height - y - 1) * p->width + x;
+                const double delta[3] = {(double)(d_film[3 * idx] * inv_spp), (double)(d_film[3 * idx + 1] * inv_spp),
+                                         (double)(d_film[3 * idx + 2] * inv_spp)};
+                for (int k = 0; k < p->spp; ++k) {
+                    Sampler smp{xx3((uint32_t)x, (uint32_t)y, (uint32_t)(p->frame_id * p->spp + k))};
+                    V2 uPix = smp.next2();
+                    V2 uLens = smp.next2();
+                    Ray r = generatePrimary(*s, p->width, p->height, x, y, uPix, uLens);
+                    liPTGrad(*s, r, smp, p->max_depth, p->rr_depth, delta, gm[w].data(), gl[w].data());
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nt; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+    for (size_t i = 0; i < nm; ++i) { d_mat[i] = 0.0; for (int w = 0; w < nt; ++w) d_mat[i] += gm[w][i]; }
+    for (size_t i = 0; i < nl; ++i) { d_light[i] = 0.0; for (int w = 0; w < nt; ++w) d_light[i] += gl[w][i]; }
+    return 0;
+}
+
+int orc_pbr_grad_d(const double* alb, double alpha, double metallic, const double* wo, const double* wi,
+                   double* f, double* d_albedo, double* d_alpha) {
+    pbrFD(alb, alpha, metallic, wo, wi, f, f + 3);
+    pbrGradD(alb, alpha, metallic, wo, wi, d_albedo, d_alpha, d_alpha + 3);
     return 0;
 }
 

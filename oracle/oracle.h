@@ -107,6 +107,23 @@ int orc_render(orc_scene* s, int32_t integrator, const orc_pt_params* p, int32_t
 int orc_trace_sample(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t sample_id,
                      float* rgb);
 
+/* Differentiable rendering: replace material albedo + roughness (4 floats
+ * each) and light emission (3 floats each); either array may be NULL. */
+int orc_set_params(orc_scene* s, const float* mat_params, int32_t n_mat, const float* light_emission,
+                   int32_t n_light);
+/* FP64 adjoint of the path-traced frame: gradients of sum(d_film * film) with
+ * respect to material albedo rgb + roughness (4 doubles each) and light
+ * emission (3 doubles each), over the pixel window (empty window: the whole
+ * film), with sampled directions, light samples, Russian roulette, pdfs and
+ * MIS weights held fixed (the definition the HIP adjoint implements). */
+int orc_adjoint_pt(orc_scene* s, const orc_pt_params* p, const float* d_film, int32_t n_threads, int32_t x0,
+                   int32_t y0, int32_t x1, int32_t y1, double* d_mat, double* d_light);
+/* PBRMaterial.Eval (PBR.fs:112-124) in double and its analytic derivatives,
+ * for checking them numerically: f[0..2] = f, f[3] = pdf; d_albedo[c] =
+ * d f_c / d albedo_c; d_alpha[0..2] = d f / d alpha, d_alpha[3] = d pdf / d alpha. */
+int orc_pbr_grad_d(const double* albedo, double alpha, double metallic, const double* wo, const double* wi,
+                   double* f, double* d_albedo, double* d_alpha);
+
 /* Camera ray of sample sample_id of pixel (x,y) (Integrator.fs:35-39,
  * Camera.fs:12-20): origin in out[0..2], direction in out[3..5]. */
 int orc_camera_ray(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t sample_id, float* out6);

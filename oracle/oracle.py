@@ -64,6 +64,11 @@ class Oracle:
         L.orc_trace_sample.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, C.c_int32, C.c_int32, f32p]
         L.orc_first_hit.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p]
         L.orc_camera_ray.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), C.c_int32, C.c_int32, C.c_int32, f32p]
+        f64p = C.POINTER(C.c_double)
+        L.orc_set_params.argtypes = [C.c_void_p, f32p, C.c_int32, f32p, C.c_int32]
+        L.orc_adjoint_pt.argtypes = [C.c_void_p, C.POINTER(abi.PTParams), f32p, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32, C.c_int32, f64p, f64p]
+        L.orc_pbr_grad_d.argtypes = [f64p, C.c_double, C.c_double, f64p, f64p, f64p, f64p, f64p]
         L.orc_render_pssmlt.argtypes = [C.c_void_p, C.POINTER(abi.MLTParams), C.c_int32, f32p,
                                         C.POINTER(abi.MLTStats), C.c_int32, C.c_int32, f32p, i32p]
         L.orc_scene_export.argtypes = [C.c_void_p, C.POINTER(abi.BVHNode), i32p, i32p, i32p, f32p, f32p,
@@ -88,6 +93,18 @@ class Oracle:
         s = C.c_uint32(state)
         out = [self.lib.orc_lcg(C.byref(s)) for _ in range(n)]
         return np.array(out, np.float32), s.value
+
+    def pbr_grad(self, albedo, alpha, metallic, wo, wi):
+        """PBRMaterial.Eval in double: (f + [pdf], d f / d albedo, d f / d alpha + [d pdf / d alpha])."""
+        f64p = C.POINTER(C.c_double)
+        a = np.ascontiguousarray(albedo, np.float64)
+        o = np.ascontiguousarray(wo, np.float64)
+        i = np.ascontiguousarray(wi, np.float64)
+        f, da, dal = np.zeros(4), np.zeros(3), np.zeros(4)
+        self._check(self.lib.orc_pbr_grad_d(a.ctypes.data_as(f64p), float(alpha), float(metallic),
+                                            o.ctypes.data_as(f64p), i.ctypes.data_as(f64p), f.ctypes.data_as(f64p),
+                                            da.ctypes.data_as(f64p), dal.ctypes.data_as(f64p)))
+        return f, da, dal
 
     def erfinv(self, x):
         return np.float32(self.lib.orc_erfinv(float(x)))
@@ -168,6 +185,25 @@ class OracleScene:
         out = np.zeros(3, np.float32)
         self.o._check(self.o.lib.orc_trace_sample(self.h, C.byref(params), x, y, k, _ptr(out, f32p)))
         return out
+
+    def set_params(self, mat_params=None, light_emission=None):
+        mp = None if mat_params is None else np.ascontiguousarray(mat_params, np.float32).reshape(-1, 4)
+        le = None if light_emission is None else np.ascontiguousarray(light_emission, np.float32).reshape(-1, 3)
+        self.o._check(self.o.lib.orc_set_params(self.h, None if mp is None else _ptr(mp, f32p),
+                                                0 if mp is None else len(mp),
+                                                None if le is None else _ptr(le, f32p), 0 if le is None else len(le)))
+
+    def adjoint_pt(self, params, d_film, threads: int = 0, window=None):
+        """FP64 gradients of sum(d_film * film): ((n_mat, 4), (n_light, 3))."""
+        threads = threads or os.cpu_count() or 1
+        df = np.ascontiguousarray(d_film, np.float32).reshape(-1)
+        dm = np.zeros((max(1, len(self.sc.materials)), 4), np.float64)
+        dl = np.zeros((max(1, len(self.sc.lights)), 3), np.float64)
+        x0, y0, x1, y1 = window if window else (0, 0, 0, 0)
+        f64p = C.POINTER(C.c_double)
+        self.o._check(self.o.lib.orc_adjoint_pt(self.h, C.byref(params), _ptr(df, f32p), threads, x0, y0, x1, y1,
+                                                dm.ctypes.data_as(f64p), dl.ctypes.data_as(f64p)))
+        return dm[:len(self.sc.materials)], dl[:len(self.sc.lights)]
 
     def camera_ray(self, params, x, y, sample_id=0):
         out = np.zeros(6, np.float32)

@@ -1,0 +1,102 @@
+"""The oracle's FP64 adjoint (the checker of the HIP path-replay adjoint).
+
+* PBR derivatives: the analytic d f / d albedo and d f / d alpha of the
+  double-precision PBRMaterial.Eval agree with central differences of it, and
+  that double evaluation agrees with the float32 one pinned in
+  tests/test_materials.py.
+* Whole-image gradients: with Russian roulette off, the estimator is a smooth
+  function of albedo and emission at fixed random numbers, so the adjoint
+  must equal central finite differences with common random numbers (h=1e-3)
+  within 1%.  Roughness moves sampled directions (GGX half-vector sampling),
+  so it is checked at max-depth 1, where it enters only through the NEE
+  term f / (pdf_b + pdf_l) at a fixed light sample (value and MIS weight).
+"""
+import ctypes as C
+
+import numpy as np
+
+from barnacle_amd import abi, configs, scene
+
+
+def test_pbr_derivatives_match_central_differences(orc):
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        alb = rng.uniform(0.05, 0.95, 3)
+        alpha = rng.uniform(0.01, 0.9)
+        met = rng.choice([0.0, 1.0, rng.uniform()])
+        wo = rng.normal(size=3); wo[2] = abs(wo[2]) + 0.05; wo /= np.linalg.norm(wo)
+        wi = rng.normal(size=3); wi[2] = abs(wi[2]) + 0.05; wi /= np.linalg.norm(wi)
+        f, da, dal = orc.pbr_grad(alb, alpha, met, wo, wi)
+        h = 1e-6
+        for c in range(3):
+            a1, a2 = alb.copy(), alb.copy()
+            a1[c] += h; a2[c] -= h
+            num = (orc.pbr_grad(a1, alpha, met, wo, wi)[0][c] - orc.pbr_grad(a2, alpha, met, wo, wi)[0][c]) / (2 * h)
+            assert abs(num - da[c]) <= 1e-6 * max(1.0, abs(num))
+        # f and the pdf (which the MIS weights differentiate)
+        num = (orc.pbr_grad(alb, alpha + h, met, wo, wi)[0] - orc.pbr_grad(alb, alpha - h, met, wo, wi)[0]) / (2 * h)
+        assert np.allclose(num, dal, rtol=1e-5, atol=1e-7 * max(1.0, np.abs(num).max()))
+        # the double evaluation is the float32 one
+        m = abi.Material(3, (C.c_float * 3)(*alb), 1.5, float(np.sqrt(alpha)), float(met))
+        f32 = orc.material_eval(m, wo.astype(np.float32), wi.astype(np.float32))
+        assert np.allclose(f32, f, rtol=1e-4, atol=1e-6)
+
+
+def small_scene(max_depth, rr_depth, spp=4, res=20, patches=12):
+    info = configs.config3(width=res, height=res, spp=spp, max_depth=max_depth, freq=6, n_patches=patches)
+    info["integrator"]["rr-depth"] = rr_depth
+    sc = scene.load_scene(info)
+    return info, sc
+
+
+def fd_check(orc, sc, info, which, idx, h=1e-3, threads=8):
+    osc = orc.scene(sc)
+    p = sc.pt_params()
+    (mat, light), _, _ = configs.config3_params(sc, info)
+    rng = np.random.default_rng(11)
+    d_film = rng.normal(size=(p.width * p.height, 3)).astype(np.float32)
+    # paths whose light sample lands on their own emissive patch can give NaN
+    # radiance (as in the reference); their pixels carry no adjoint
+    base, _ = osc.render(p, threads=threads)
+    finite = np.isfinite(base).all(axis=1)
+    d_film[~finite] = 0.0
+    dm, dl = osc.adjoint_pt(p, d_film, threads=threads)
+    out = []
+    for (k, c) in idx:
+        vals = []
+        for sgn in (1, -1):
+            m2, l2 = mat.copy(), light.copy()
+            if which == "mat":
+                m2[k, c] += sgn * h
+            else:
+                l2[k, c] += sgn * h
+            osc.set_params(m2, l2)
+            f, _ = osc.render(p, threads=threads)
+            assert np.array_equal(np.isfinite(f).all(axis=1), finite)
+            vals.append(float(np.sum(np.where(finite[:, None], f, 0).astype(np.float64) * d_film)))
+        osc.set_params(mat, light)
+        fd = (vals[0] - vals[1]) / (2 * h)
+        ad = dm[k, c] if which == "mat" else dl[k, c]
+        out.append((fd, ad))
+    return out
+
+
+def test_albedo_and_emission_gradients_match_fd(orc):
+    info, sc = small_scene(max_depth=4, rr_depth=4)
+    p = info["x-patches"]
+    fm, fl = p["first_material"], p["first_light"]
+    for which, idx in (("mat", [(0, 0), (1, 1), (2, 2), (fm, 0), (fm + 3, 1), (fm + 7, 2)]),
+                       ("light", [(0, 0), (0, 2), (fl, 0), (fl + 5, 1)])):
+        for fd, ad in fd_check(orc, sc, info, which, idx):
+            assert abs(fd - ad) <= 0.01 * abs(fd) + 1e-6, (which, fd, ad)
+
+
+def test_roughness_gradient_through_nee_matches_fd(orc):
+    info, sc = small_scene(max_depth=1, rr_depth=1, spp=8)
+    fm = info["x-patches"]["first_material"]
+    # a wider step: the loss is smooth in roughness here, and float32 films
+    # leave ~1e-5 of noise in a 1e-3 difference quotient
+    res = fd_check(orc, sc, info, "mat", [(fm + k, 3) for k in range(0, 12, 3)], h=3e-3)
+    assert any(abs(fd) > 1e-3 for fd, _ in res)
+    for fd, ad in res:
+        assert abs(fd - ad) <= 0.01 * abs(fd) + 1e-5, (fd, ad)
