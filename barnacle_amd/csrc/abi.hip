@@ -457,6 +457,7 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
     L.n_pix = (uint32_t)(p.width * p.height);
     L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
+    c->fwd.valid = false;
     for (int s0 = 0; s0 < p.spp; s0 += ns) {
         it.s0 = s0;
         it.ns = std::min(ns, p.spp - s0);
@@ -464,6 +465,13 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
         L.item = it;
         run_round(c, false, normal ? bnd::Integrator::Normal : bnd::Integrator::Path, L, P, ctl, B,
                   (uint32_t)it.path_count, it.max_depth, stats != nullptr, &hook, nullptr);
+    }
+    if (!normal && ns >= p.spp) {                           // one round: L_out holds the frame
+        c->fwd.valid = true;
+        c->fwd.scene = &sc;
+        c->fwd.gen = sc.gen;
+        c->fwd.p = p;
+        c->fwd.slots = slots;
     }
     hip_check(hipGetLastError(), "kernel launch");
     if (film_host) hip_check(hipMemcpyAsync(film_host, c->film.p, film_n * 4, hipMemcpyDeviceToHost, s), "film D2H");
@@ -508,6 +516,7 @@ void set_params(Ctx* c, SceneRec& sc, const float* mp, int n_mat, const float* l
     }
     hip_check(hipStreamSynchronize(c->stream), "set params");
     c->graphs.clear();
+    ++sc.gen;
 }
 
 // Gradient of sum(d_film * film) for the frame bn_render_pt renders with the
@@ -564,16 +573,26 @@ void backward_pt(Ctx* c, SceneRec& sc, const bn_pt_params& p, const float* d_fil
     B.adj.d_film = (const float*)c->a_dfilm.p;
     B.adj.g_mat = (float*)c->a_gmat.p;
     B.adj.g_light = (float*)c->a_glight.p;
+    // The forward pass of the same frame may still be in L_out (the last
+    // bn_render_pt, one round, same scene parameters): then only replay.
+    const bool cached = c->fwd.valid && c->fwd.scene == &sc && c->fwd.gen == sc.gen && ns >= p.spp &&
+                        c->fwd.slots == slots && std::memcmp(&c->fwd.p, &p, sizeof(p)) == 0;
     for (int s0 = 0; s0 < p.spp; s0 += ns) {
         it.s0 = s0;
         it.ns = std::min(ns, p.spp - s0);
         it.path_count = it.ns * it.p_pad;
         L.item = it;
-        run_round(c, false, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, false,
-                  &hook, nullptr);
+        if (!cached)
+            run_round(c, false, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, false,
+                      &hook, nullptr);
         run_round(c, false, bnd::Integrator::Adjoint, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, false,
                   &hook, nullptr);
     }
+    c->fwd.valid = ns >= p.spp;                             // L_out still holds this frame's forward
+    c->fwd.scene = &sc;
+    c->fwd.gen = sc.gen;
+    c->fwd.p = p;
+    c->fwd.slots = slots;
     hip_check(hipGetLastError(), "kernel launch");
     std::vector<float> gm((size_t)std::max(1, n_mat) * 4), gl((size_t)std::max(1, n_light) * 4);
     hip_check(hipMemcpyAsync(gm.data(), c->a_gmat.p, gm.size() * 4, hipMemcpyDeviceToHost, s), "gradient D2H");
@@ -623,6 +642,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     c->film.ensure(film_total * 4);
     c->ctl.ensure(sizeof(bnd::Ctl));
     c->stats.ensure(8 * sizeof(unsigned long long));
+    c->fwd.valid = false;                                   // L_out is reused below
     c->b_scenes.upload(scenes.data(), scenes.size() * sizeof(bnd::DevScene), s);
     c->b_items.ensure((size_t)n * sizeof(bnd::RoundItem));
     c->b_hdr.ensure(sizeof(bnd::RoundHeader));
@@ -1111,8 +1131,8 @@ int bn_scene_release(bn_ctx* ctx, int32_t scene) {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
         SceneRec& s = get_scene(c, scene);
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        if (c->fwd.scene == &s) c->fwd.valid = false;
         c->scenes[scene] = std::make_unique<SceneRec>();
-        (void)s;
     });
 }
 

@@ -44,6 +44,7 @@ struct SceneRec {
     bn_camera camera{};
     float c2w[16] = {};
     bnd::DevScene dev{};
+    uint64_t gen = 0;                        // bumped by bn_set_params
     std::vector<bn_material> mats_host;     // what bn_set_params edits
     std::vector<bn_light> lights_host;
     std::shared_ptr<bnh::HostScene> host;   // set by bn_scene_build
@@ -88,6 +89,9 @@ struct Ctx {
     // path-replay adjoint: per-slot terms, gradients, film adjoint
     DevBuf a_delta, a_pend_a, a_pend_r, a_nee_da, a_nee_dr, a_gmat, a_glight, a_dfilm;
     size_t adj_cap = 0;
+    // L_out holds the radiance of every path of this single-round frame
+    // (what bn_backward_pt replays against), when valid.
+    struct { bool valid; const SceneRec* scene; uint64_t gen; bn_pt_params p; size_t slots; } fwd{};
     size_t spill_stride = 0;
     DevBuf out_a, out_b, out_c;
     KernelTimer timer;

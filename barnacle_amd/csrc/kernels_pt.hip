@@ -699,14 +699,13 @@ __global__ __launch_bounds__(256) void k_shade_adj(Work<false> W, PathBuf P, Adj
     }
 }
 
-// The shadow ray of a replayed bounce: when unoccluded, the NEE contribution
-// leaves the radiance still to come and adds its gradients (the vertex's
-// material through f, the sampled light through its emission); then the
-// vertex's BSDF-sample term is applied.
+// The shadow rays of a replayed bounce only record their visibility (in
+// nee_a.w): gradient sums belong in a kernel whose lanes are all busy, not in
+// the traversal loop.
 template <int K>
-__global__ __launch_bounds__(256) void k_trace_shadow_adj(Work<false> W, PathBuf P, AdjBuf A, Ctl* ctl,
-                                                          const uint32_t* shq, uint32_t* spill,
-                                                          size_t spill_stride, uint32_t refill, uint32_t* seg) {
+__global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
+                                                          uint32_t* spill, size_t spill_stride, uint32_t refill,
+                                                          uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
@@ -719,39 +718,52 @@ __global__ __launch_bounds__(256) void k_trace_shadow_adj(Work<false> W, PathBuf
             if (!(o4.w == o4.w)) tr.st.sp = 0;
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
-            int mat_key = -1, light_key = -1;
-            float4 g_m = make_float4(0, 0, 0, 0), g_l = make_float4(0, 0, 0, 0);
-            if (fin) {
-                const uint32_t slot = shq[i];
-                V3 L = xyz(P.L[slot]);
-                const V3 dlt = xyz(A.delta[slot]);
-                V3 gm = mk(0, 0, 0);
-                float gr = 0.0f;
-                if (!found) {
-                    const float4 a = P.nee_a[slot], b = P.nee_b[slot];
-                    const float4 da = A.nee_da[slot], dr = A.nee_dr[slot];
-                    const V3 c = xyz(a) * xyz(b);
-                    L = L - c;
-                    gm = dlt * xyz(da) * xyz(b);
-                    // d (f Le / (pdf_b + pdf_l)) / d rough = f' Le / S - c pdf_b' / S
-                    gr = dot(dlt, xyz(dr) * xyz(b) - c * P.sh_d[slot].w);
-                    mat_key = __float_as_int(da.w);
-                    light_key = __float_as_int(dr.w);
-                    g_l = f4(dlt * xyz(a) * b.w, 0.0f);
-                }
-                const float4 pa = A.pend_a[slot];
-                const int pend = __float_as_int(pa.w);
-                if (pend >= 0) {
-                    gm = gm + dlt * xyz(pa) * L;
-                    gr += dot(dlt * xyz(A.pend_r[slot]), L);
-                    mat_key = pend;
-                }
-                g_m = f4(gm, gr);
-                P.L[slot] = f4(L, 0.0f);
-            }
-            grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
-            grad_accum(A.g_light, light_key, g_l, light_key >= 0);
+            if (fin) P.nee_a[shq[i]].w = found ? 0.0f : 1.0f;
         });
+}
+
+// Then, per shadow ray: when unoccluded, the NEE contribution leaves the
+// radiance still to come and adds its gradients (the vertex's material
+// through f and the NEE MIS weight, the sampled light through its emission);
+// then the vertex's BSDF-sample term is applied.
+__global__ __launch_bounds__(256) void k_nee_adj(PathBuf P, AdjBuf A, const Ctl* ctl, const uint32_t* shq) {
+    const uint32_t count = ctl->sh_count;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        int mat_key = -1, light_key = -1;
+        float4 g_m = make_float4(0, 0, 0, 0), g_l = make_float4(0, 0, 0, 0);
+        if (i < count) {
+            const uint32_t slot = shq[i];
+            V3 L = xyz(P.L[slot]);
+            const V3 dlt = xyz(A.delta[slot]);
+            const float4 a = P.nee_a[slot];
+            V3 gm = mk(0, 0, 0);
+            float gr = 0.0f;
+            if (a.w != 0.0f) {
+                const float4 b = P.nee_b[slot];
+                const float4 da = A.nee_da[slot], dr = A.nee_dr[slot];
+                const V3 c = xyz(a) * xyz(b);
+                L = L - c;
+                gm = dlt * xyz(da) * xyz(b);
+                // d (f Le / (pdf_b + pdf_l)) / d rough = f' Le / S - c pdf_b' / S
+                gr = dot(dlt, xyz(dr) * xyz(b) - c * P.sh_d[slot].w);
+                mat_key = __float_as_int(da.w);
+                light_key = __float_as_int(dr.w);
+                g_l = f4(dlt * xyz(a) * b.w, 0.0f);
+            }
+            const float4 pa = A.pend_a[slot];
+            const int pend = __float_as_int(pa.w);
+            if (pend >= 0) {
+                gm = gm + dlt * xyz(pa) * L;
+                gr += dot(dlt * xyz(A.pend_r[slot]), L);
+                mat_key = pend;
+            }
+            g_m = f4(gm, gr);
+            P.L[slot] = f4(L, 0.0f);
+        }
+        grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
+        grad_accum(A.g_light, light_key, g_l, light_key >= 0);
+    }
 }
 
 // -------------------------------------------------------------------- film
@@ -923,14 +935,16 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
                 { Bracket b(hook, "shade_adj", st);
                   hipLaunchKernelGGL(k_shade_adj, dim3(L.grid_shade), dim3(256), 0, st, W, P, B.adj, ctl, B.bins,
                                      B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
-                { Bracket b(hook, "trace_shadow_adj", st);
+                { Bracket b(hook, "trace_shadow", st);
                   uint32_t* seg = B.fetch_seg + kFetchSegs * kSegStride;
                   if (B.stack_k == 16)
-                      hipLaunchKernelGGL(k_trace_shadow_adj<16>, dim3(L.grid_trace), dim3(256), 0, st, W, P, B.adj,
-                                         ctl, B.shq, B.spill, B.spill_stride, L.refill, seg);
+                      hipLaunchKernelGGL(k_trace_shadow_vis<16>, dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
+                                         B.shq, B.spill, B.spill_stride, L.refill, seg);
                   else
-                      hipLaunchKernelGGL(k_trace_shadow_adj<32>, dim3(L.grid_trace), dim3(256), 0, st, W, P, B.adj,
-                                         ctl, B.shq, B.spill, B.spill_stride, L.refill, seg); }
+                      hipLaunchKernelGGL(k_trace_shadow_vis<32>, dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
+                                         B.shq, B.spill, B.spill_stride, L.refill, seg); }
+                { Bracket b(hook, "nee_adj", st);
+                  hipLaunchKernelGGL(k_nee_adj, dim3(L.grid_shade), dim3(256), 0, st, P, B.adj, ctl, B.shq); }
             }
         } else {
             { Bracket b(hook, "shade", st);

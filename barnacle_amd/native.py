@@ -175,6 +175,7 @@ class DeviceScene:
         sid = C.c_int32()
         check(ctx.lib.bn_scene_build(ctx.h, C.byref(desc), C.byref(sid)))
         self.id = sid.value
+        self.param_gen = 0
 
     def render_pt(self, params: abi.PTParams, stats: bool = False):
         """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters."""
@@ -221,6 +222,7 @@ class DeviceScene:
         check(self.ctx.lib.bn_set_params(self.ctx.h, self.id, None if mp is None else _p(mp, f32p),
                                          0 if mp is None else len(mp), None if le is None else _p(le, f32p),
                                          0 if le is None else len(le)))
+        self.param_gen += 1
 
     def backward_pt(self, params: abi.PTParams, d_film):
         """Gradients of sum(d_film * film): ((n_mat, 4), (n_light, 3))."""

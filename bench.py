@@ -12,6 +12,9 @@ already resident in HBM.  Prints one JSON line:
                  FETCH_SIZE / WRITE_SIZE summary
   batched        32 seeded scenes rendered one after the other (plain) and as
                  one batch; speedup = plain time / batched time
+  adjoint        inverse rendering of the 4096-patch scene: Adam iterations
+                 (forward + path-replay backward) per second, and their time
+                 over the forward render's
   cpu_baseline   the CPU oracle (C++ restatement of the F# renderer) on a
                  bounded pixel window of the same frame, all host threads up to 16
   parity         GPU vs oracle on a 128x128x2spp render of the same scene
@@ -98,6 +101,35 @@ def batched(ctx, n=32):
             "films_bit_equal": equal, "scene_setup_s": setup}
 
 
+def adjoint(ctx, steps=12):
+    """config3 inverse rendering: Adam iterations of path-replay gradients."""
+    import torch  # noqa: F401  (the autograd wrapper runs on host tensors)
+    from barnacle_amd import diff
+    info = configs.config3()
+    sc = scene.load_scene(info)
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    (tm, tl), _, _ = configs.config3_params(sc, info)
+    ds.set_params(tm, tl)
+    target = torch.from_numpy(ds.render_pt(p))
+    fwd = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ds.render_pt(p)
+        fwd.append(time.perf_counter() - t0)
+    stamps = []
+    _, hist = diff.optimise(ds, info, sc, p, steps=steps, target=target,
+                            callback=lambda i, l: stamps.append(time.perf_counter()))
+    iters = np.diff(np.asarray(stamps))[1:]                 # steady state: the first step allocates
+    it_s = float(np.median(iters))
+    f_s = float(np.median(fwd))
+    return {"config": "config3: 4096-patch GGX sphere in the Cornell box, 512x512, 32 spp, depth 8, "
+                      "4096 materials x (albedo rgb, roughness) + 4096 lights x emission rgb",
+            "iters_per_s": 1.0 / it_s, "iter_ms": it_s * 1e3, "forward_ms": f_s * 1e3,
+            "iter_over_forward": it_s / f_s, "steps": steps, "loss_first": hist[0], "loss_last": hist[-1],
+            "nonfinite_pixel_fraction": float((~torch.isfinite(target).all(dim=1)).float().mean())}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device hipMemcpy (the runtime's copy kernel) of nbytes."""
     import ctypes
@@ -136,6 +168,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--spp", type=int, default=64)
     args = ap.parse_args()
@@ -208,6 +241,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched(ctx)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint(ctx)
     if not args.no_parity:
         out["parity"] = parity(ctx)
     if args.copy_calibration:
