@@ -15,6 +15,7 @@ already resident in HBM.  Prints one JSON line:
   adjoint        inverse rendering of the 4096-patch scene: Adam iterations
                  (forward + path-replay backward) per second, and their time
                  over the forward render's
+  pssmlt         the caustic scene: mutations per second and acceptance
   cpu_baseline   the CPU oracle (C++ restatement of the F# renderer) on a
                  bounded pixel window of the same frame, all host threads up to 16
   parity         GPU vs oracle on a 128x128x2spp render of the same scene
@@ -130,6 +131,23 @@ def adjoint(ctx, steps=12):
             "nonfinite_pixel_fraction": float((~torch.isfinite(target).all(dim=1)).float().mean())}
 
 
+def pssmlt(ctx):
+    """config4: PSSMLT on the caustic scene, 2^20 bootstrap paths, 2^20 chains x 256 mutations."""
+    sc = scene.load_scene(configs.caustic_scene())
+    ds = ctx.scene(sc)
+    p = sc.mlt_params()
+    ds.render_pssmlt(sc.mlt_params(spp=4))
+    t0 = time.perf_counter()
+    film, st = ds.render_pssmlt(p)
+    el = time.perf_counter() - t0
+    return {"config": "config4: Cornell box + dielectric sphere (IOR 1.5) + GGX metal sphere (roughness 0.1), "
+                      "1024x1024, depth 16, 2^20 bootstrap, 2^20 chains x 256 mutations, large step 0.3, "
+                      "sigma 1/64",
+            "seconds": el, "mmutations_per_s": st["proposed"] / el / 1e6,
+            "acceptance": st["accepted"] / max(1, st["proposed"]), "b": float(st["b"]),
+            "nan_pixel_fraction": float((~np.isfinite(film).all(axis=1)).mean())}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device hipMemcpy (the runtime's copy kernel) of nbytes."""
     import ctypes
@@ -169,6 +187,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-pssmlt", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--spp", type=int, default=64)
     args = ap.parse_args()
@@ -243,6 +262,8 @@ def main():
         out["batched"] = batched(ctx)
     if not args.no_adjoint:
         out["adjoint"] = adjoint(ctx)
+    if not args.no_pssmlt:
+        out["pssmlt"] = pssmlt(ctx)
     if not args.no_parity:
         out["parity"] = parity(ctx)
     if args.copy_calibration:

@@ -86,3 +86,28 @@ def test_adjoint_4096_patches_window_matches_oracle(ctx, orc):
         scale = np.abs(o).max()
         assert scale > 0
         assert np.linalg.norm(g - o) <= 5e-3 * np.linalg.norm(o)
+
+
+def test_autograd_function_and_optimisation(ctx):
+    """RenderPT.apply's gradient equals bn_backward_pt's, and Adam on the
+    masked L2 loss lowers it."""
+    torch = pytest.importorskip("torch")
+    from barnacle_amd import diff
+    info, sc = make(max_depth=4, rr_depth=3, spp=4, res=24)
+    p = sc.pt_params()
+    ds = ctx.scene(sc)
+    (tm, tl), (m0, l0), _ = configs.config3_params(sc, info)
+    ds.set_params(tm, tl)
+    target = torch.from_numpy(ds.render_pt(p))
+    mat = torch.tensor(m0, requires_grad=True)
+    light = torch.tensor(l0, requires_grad=True)
+    film = diff.RenderPT.apply(mat, light, ds, p)
+    loss = diff.masked_l2(film, target)
+    loss.backward()
+    ok = torch.isfinite(film).all(dim=1) & torch.isfinite(target).all(dim=1)
+    d = torch.where(ok[:, None], 2.0 * (film - target) / (int(ok.sum()) * 3), torch.zeros_like(film))
+    gm, gl = ds.backward_pt(p, d.detach().numpy().astype(np.float32))
+    assert np.allclose(mat.grad.numpy(), gm, rtol=1e-3, atol=1e-6 * np.abs(gm).max())
+    assert np.allclose(light.grad.numpy(), gl, rtol=1e-3, atol=1e-6 * np.abs(gl).max())
+    _, hist = diff.optimise(ds, info, sc, p, steps=8, lr=5e-2, target=target)
+    assert hist[-1] < hist[0]
