@@ -803,7 +803,7 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
         if ((size_t)v >= slots_total) break;          // larger chunks change nothing
     }
     if (chunks.empty()) chunks.push_back(chunk);
-    std::vector<Knob> knobs = {{&blocks, {2, 3, 4, 5}}, {&refill, {32, 40, 48, 56}}, {&stack_k, {16, 32}},
+    std::vector<Knob> knobs = {{&blocks, {3, 4, 5, 6, 8}}, {&refill, {32, 40, 48, 56}}, {&stack_k, {16, 32}},
                                {&chunk, chunks}};
     apply();
     double best = measure();
@@ -1240,7 +1240,7 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
             if (value < 64 || value > ((int64_t)1 << 30)) throw Error(BN_E_INVALID, "chunk_slots out of range");
             c->chunk_slots = (size_t)value;
         } else if (k == "trace_blocks_per_cu") {
-            if (value < 1 || value > 5) throw Error(BN_E_INVALID, "trace_blocks_per_cu must be in [1, 5]");
+            if (value < 1 || value > 8) throw Error(BN_E_INVALID, "trace_blocks_per_cu must be in [1, 8]");
             c->trace_blocks_per_cu = (int)value;
         } else if (k == "mlt_chunk") {
             if (value < 64 || value > ((int64_t)1 << 26)) throw Error(BN_E_INVALID, "mlt_chunk out of range");

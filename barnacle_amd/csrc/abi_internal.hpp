@@ -76,7 +76,7 @@ struct Ctx {
     int device = 0;
     int n_cu = 1;
     hipStream_t stream = nullptr;
-    int trace_blocks_per_cu = 4;
+    int trace_blocks_per_cu = 6;
     int refill = 48;                      // idle lanes that trigger a refill
     size_t chunk_slots = (size_t)64 << 20;       // path slots in flight
     size_t window_paths = 0;                     // paths per round; 0: as many as slots
@@ -97,7 +97,7 @@ struct Ctx {
     KernelTimer timer;
     // batched rounds
     bool use_graph = true;
-    int stack_k = 32;                      // LDS traversal stack window
+    int stack_k = 16;                      // LDS traversal stack window
     int batch_pack = 0;                    // 0 item-major, 1 fair share
     DevBuf b_scenes, b_items, b_hdr;
     std::map<std::string, GraphExec> graphs;   // captured bounce loops by configuration

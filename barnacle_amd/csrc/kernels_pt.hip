@@ -279,7 +279,7 @@ __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uin
             continue;
         }
         bool fin = false, found = false;
-        if (active) fin = !tr.template step<ANY, COUNT>(tc, found);
+        if (active) fin = COUNT ? !tr.template step<ANY, COUNT>(tc, found) : !tr.template step_fast<ANY>(found);
         if (__ballot(fin) != 0) {
             if (fin) active = false;
             done(tr, item, fin, found);
@@ -290,8 +290,10 @@ __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uin
 // BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  The
 // material bin of the hit (kind of the instance's material; light-only
 // instances in the last bin, -1 for a miss) travels in the hit record.
+// With a 16-entry LDS stack window six blocks fit a CU's LDS; the register
+// budget is then set for six waves per SIMD (a little spilling included).
 template <bool COUNT, bool BATCH, int K>
-__global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
+__global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* spill, size_t spill_stride,
                                                        uint32_t refill, unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(Work<BATCH> W, PathBuf P,
             const float4 o4 = P.o[slot];
             tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.d[slot])}, kInf);
             // a placeholder path (padding pixel, max-depth 0) misses at once
-            if (__float_as_int(o4.w) < 0) tr.st.sp = 0;
+            if (__float_as_int(o4.w) < 0) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool) {
             if (fin) {
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 // aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
 // an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
 template <bool COUNT, bool BATCH, int K>
-__global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
+__global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -514,7 +516,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(Work<BATCH> W, PathBuf P, 
             const float4 o4 = P.sh_o[slot];
             tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
             // a NaN t max fails every slab test of the reference: empty traversal
-            if (!(o4.w == o4.w)) tr.st.sp = 0;
+            if (!(o4.w == o4.w)) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
             if (fin && !found) {
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
             tr.start(W.scene(0), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
-            if (!(o4.w == o4.w)) tr.st.sp = 0;
+            if (!(o4.w == o4.w)) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
             if (fin) P.nee_a[shq[i]].w = found ? 0.0f : 1.0f;

@@ -36,6 +36,9 @@ __device__ __forceinline__ bool finite3(V3 v) {
 }
 
 enum : int { kItemNone = 0, kItemTNode = 1, kItemBNode = 2, kItemTri = 3, kItemInst = 4 };
+constexpr uint32_t kNoNode = 0xffffffffu;
+constexpr uint32_t kNeedTest = 0x80000000u;   // node whose box is still to be tested
+constexpr uint32_t kNodeMask = 0x7fffffffu;
 
 // The geometry a lane traverses (per lane, so a batched launch can mix scenes).
 struct TraceScene {
@@ -57,6 +60,7 @@ struct Traversal {
     bool exact;          // take the NaN-exact slab path
     // traversal position
     LaneStack<K> st;
+    uint32_t nxt;        // node to take before the stack (kNoNode: none); kNeedTest: box not tested yet
     int blas_base;       // stack depth at BLAS entry, -1 in the TLAS
     int tl_cur, tl_end;  // instances left in the current TLAS leaf
     int bl_cur, bl_end;  // triangles left in the current BLAS leaf (global indices)
@@ -72,12 +76,15 @@ struct Traversal {
         hit_inst = -1; hit_prim = -1;
         exact = exact_scene || !finite3(wo) || !finite3(winv);
         st.sp = 0;
-        st.push(0);
+        nxt = kNeedTest;                 // the TLAS root
         blas_base = -1;
         tl_cur = tl_end = 0;
         bl_cur = bl_end = 0;
         blas_root = tri_base = cur_inst = 0;
     }
+
+    // No traversal at all (placeholder rays, NaN t max).
+    __device__ __forceinline__ void empty() { st.sp = 0; nxt = kNoNode; }
 
     __device__ __forceinline__ bool box(V3 lo, V3 hi) const {
         return exact ? slab(lo, hi, co, cinv, t) : slab_fast(lo, hi, co, cinv, t);
@@ -85,10 +92,13 @@ struct Traversal {
 
     // One item.  Returns false when the traversal has finished (no items left,
     // or an any-hit query found its hit); `found` reports an any-hit.
+    // The reference's visit sequence item by item (every popped node is
+    // tested); the counting kernels use it, so the counters are the
+    // reference's.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool step(TraceCount& cnt, bool& found) {
         // leave a finished BLAS
-        if (blas_base >= 0 && bl_cur >= bl_end && st.sp == blas_base) {
+        if (blas_base >= 0 && bl_cur >= bl_end && nxt == kNoNode && st.sp == blas_base) {
             blas_base = -1;
             co = wo; cd = wd; cinv = winv;
         }
@@ -96,9 +106,12 @@ struct Traversal {
         int idx = 0;
         if (blas_base >= 0) {
             if (bl_cur < bl_end) { kind = kItemTri; idx = bl_cur++; }
+            else if (nxt != kNoNode) { kind = kItemBNode; idx = (int)(nxt & kNodeMask); nxt = kNoNode; }
             else { kind = kItemBNode; idx = (int)st.pop(); }
         } else if (tl_cur < tl_end) {
             kind = kItemInst; idx = tl_cur++;
+        } else if (nxt != kNoNode) {
+            kind = kItemTNode; idx = (int)(nxt & kNodeMask); nxt = kNoNode;
         } else if (st.sp > 0) {
             kind = kItemTNode; idx = (int)st.pop();
         } else {
@@ -152,7 +165,7 @@ struct Traversal {
                 blas_root = info.x;
                 tri_base = info.y;
                 blas_base = st.sp;
-                st.push(0);
+                nxt = kNeedTest;         // the BLAS root
             }
         } else if (n.leaf) {
             if (kind == kItemTNode) { tl_cur = n.word; tl_end = n.word + n.count; }
@@ -163,6 +176,130 @@ struct Traversal {
             st.push((uint32_t)(idx + 1)); st.push((uint32_t)n.word);
         }
         return true;
+    }
+
+    // The fast traversal (do_tri / do_inst / do_node): the same hits with
+    // fewer iterations.  An interior node tests both children when it is
+    // expanded, with the t of that moment, and only the children that pass
+    // are kept.  A kept child is not re-tested when it is taken later with a
+    // smaller t.  That changes no hit:
+    // if its box would fail then, the box lies beyond the current t, and so do
+    // the boxes of every triangle and instance below it, which are still
+    // tested (Mesh.fs:204,231 pre-test every triangle; instances have their
+    // world bounds); a box that passes with the current t passed with the
+    // larger one too.  Every triangle the reference tests is tested, in the
+    // same order, so ties resolve alike.
+    // What this lane does next (leaving a finished BLAS first): kItemTri,
+    // kItemInst, kItemTNode (any node), or kItemNone when it has finished.
+    __device__ __forceinline__ int next_kind() {
+        if (blas_base >= 0 && bl_cur >= bl_end && nxt == kNoNode && st.sp == blas_base) {
+            blas_base = -1;
+            co = wo; cd = wd; cinv = winv;
+        }
+        if (blas_base >= 0 && bl_cur < bl_end) return kItemTri;
+        if (blas_base < 0 && tl_cur < tl_end) return kItemInst;
+        if (nxt != kNoNode || st.sp > (blas_base >= 0 ? blas_base : 0)) return kItemTNode;
+        return kItemNone;
+    }
+
+    // One triangle of the current BLAS leaf.  Returns false on an any-hit.
+    template <bool ANY>
+    __device__ __forceinline__ bool do_tri(bool& found) {
+        const int idx = bl_cur++;
+        const Tri tr = load_tri(G.tris, idx);
+        if (!box(vmin(vmin(tr.p0, tr.p1), tr.p2), vmax(vmax(tr.p0, tr.p1), tr.p2))) return true;
+        float tt;
+        if (tri_hit(tr, co, cd, t, tt)) {
+            if (ANY) { found = true; return false; }
+            t = tt;
+            hit_inst = cur_inst;
+            hit_prim = idx - tri_base;
+        }
+        return true;
+    }
+
+    // One instance of the current TLAS leaf: a sphere is intersected, a mesh
+    // BLAS is entered.  Returns false on an any-hit.
+    template <bool ANY>
+    __device__ __forceinline__ bool do_inst(bool& found) {
+        const int idx = tl_cur++;
+        const DevInst& I = G.insts[idx];
+        if (!box(xyz(I.lo), xyz(I.hi))) return true;
+        const Xf w2o = I.w2o;
+        const V3 oo = xf_point(w2o, wo), od = xf_dir(w2o, wd);
+        const int4 info = I.info;
+        if (info.x < 0) {
+            float tt;
+            int br;
+            if (sphere_hit(I.lo.w, oo, od, t, tt, br)) {
+                if (ANY) { found = true; return false; }
+                t = tt;
+                hit_inst = idx;
+                hit_prim = br;
+            }
+        } else {
+            co = oo; cd = od;
+            cinv = {1.0f / od.x, 1.0f / od.y, 1.0f / od.z};
+            exact = exact || !finite3(co) || !finite3(cinv);
+            cur_inst = idx;
+            blas_root = info.x;
+            tri_base = info.y;
+            blas_base = st.sp;
+            nxt = kNeedTest;
+        }
+        return true;
+    }
+
+    // One item of the fast traversal (all lanes of a wave advance together,
+    // whatever kind of item each has: measured faster than advancing only the
+    // lanes of the most common kind).  Returns false when the traversal has
+    // finished or an any-hit query found its hit.
+    template <bool ANY>
+    __device__ __forceinline__ bool step_fast(bool& found) {
+        const int kind = next_kind();
+        if (kind == kItemTri) return do_tri<ANY>(found);
+        if (kind == kItemInst) return do_inst<ANY>(found);
+        if (kind == kItemTNode) { do_node(); return true; }
+        return false;
+    }
+
+    // Expand the next node: a leaf sets its item range; an interior node
+    // tests both children with the current t and keeps those that pass (the
+    // near one in `nxt`, the far one on the stack, near first as in
+    // Extensions/Aggregate/BVH.fs:47-56 and Mesh.fs:233-240).
+    __device__ __forceinline__ void do_node() {
+        uint32_t e;
+        if (nxt != kNoNode) { e = nxt; nxt = kNoNode; }
+        else e = st.pop();
+        const float4* base = blas_base >= 0 ? G.blas + 2 * (size_t)blas_root : G.tlas;
+        const int node = (int)(e & kNodeMask);
+        const float4 b = base[2 * node + 1];
+        if (e & kNeedTest) {
+            const float4 a = base[2 * node];
+            if (!box(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y))) return;
+        }
+        const int word = __float_as_int(b.z);
+        const uint32_t f = __float_as_uint(b.w);
+        if (f & 0xff) {
+            const int count = (int)(int8_t)((f >> 16) & 0xff);
+            if (blas_base < 0) { tl_cur = word; tl_end = word + count; }
+            else { bl_cur = tri_base + word; bl_end = bl_cur + count; }
+            return;
+        }
+        const int axis = (int)(int8_t)((f >> 8) & 0xff);
+        const int l = node + 1, r = word;
+        const float4 la = base[2 * l], lb = base[2 * l + 1], ra = base[2 * r], rb = base[2 * r + 1];
+        const bool hl = box(mk(la.x, la.y, la.z), mk(la.w, lb.x, lb.y));
+        const bool hr = box(mk(ra.x, ra.y, ra.z), mk(ra.w, rb.x, rb.y));
+        const bool left_near = comp(cd, axis) > 0.0f;
+        const int near = left_near ? l : r, far = left_near ? r : l;
+        const bool hn = left_near ? hl : hr, hf = left_near ? hr : hl;
+        if (hn) {
+            nxt = (uint32_t)near;
+            if (hf) st.push((uint32_t)far);
+        } else if (hf) {
+            nxt = (uint32_t)far;
+        }
     }
 };
 

@@ -295,7 +295,7 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
  *   "refill"               idle lanes of a wave that trigger a refill (1..64)
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
- *   "stack_k"              LDS window of the traversal stack: 16 or 32 (default)
+ *   "stack_k"              LDS window of the traversal stack: 16 (default) or 32
  *   "graph"                1 (default): replay batched rounds as a HIP graph
  *   "batch_pack"           0 (default): pack rounds item-major; 1: fair share */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);

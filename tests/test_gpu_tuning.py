@@ -16,7 +16,7 @@ def mesh_job(ctx):
 
 
 SETTINGS = [
-    {"stack_k": 16},
+    {"stack_k": 32},
     {"stack_k": 32, "trace_blocks_per_cu": 2, "refill": 16},
     {"stack_k": 16, "trace_blocks_per_cu": 5, "refill": 64, "chunk_slots": 1000},
 ]
@@ -32,7 +32,7 @@ def test_launch_parameters_do_not_change_film(ctx, mesh_job, opts):
         got = ds.render_pt(p)
         films, _ = ctx.render_batch([(ds, p), (ds, p)])
     finally:
-        for k, v in {"stack_k": 32, "trace_blocks_per_cu": 4, "refill": 48, "chunk_slots": 64 << 20}.items():
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20}.items():
             ctx.set_option(k, v)
     assert np.array_equal(got.view(np.int32), ref.view(np.int32))
     for f in films:
@@ -44,7 +44,7 @@ def test_autotune(ctx, mesh_job):
     ref = ds.render_pt(p)
     try:
         r = ctx.autotune([(ds, p)])
-        assert r["trace_blocks_per_cu"] in (2, 3, 4, 5)
+        assert r["trace_blocks_per_cu"] in (3, 4, 5, 6, 8)
         assert r["refill"] in (32, 40, 48, 56)
         assert r["stack_k"] in (16, 32)
         assert r["trials"] >= 8
@@ -53,5 +53,5 @@ def test_autotune(ctx, mesh_job):
         rb = ctx.autotune([(ds, p), (ds, p)], batched=True)
         assert rb["trials"] >= 8
     finally:
-        for k, v in {"stack_k": 32, "trace_blocks_per_cu": 4, "refill": 48, "chunk_slots": 64 << 20}.items():
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20}.items():
             ctx.set_option(k, v)
