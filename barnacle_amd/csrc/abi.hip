@@ -345,6 +345,7 @@ bnd::Buffers buffers(Ctx* c) {
     B.L_out = (float4*)c->L_out.p;
     c->fetch_seg.ensure((size_t)2 * bnd::kFetchSegs * bnd::kSegStride * 4);
     B.fetch_seg = (uint32_t*)c->fetch_seg.p;
+    B.sort = c->material_sort;
     return B;
 }
 
@@ -899,7 +900,7 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
                                         (uint32_t)c->refill, gt, s); }
             { Timed tm(T, "bin", s);
               bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, nullptr,
-                              gs, s); }
+                              c->material_sort > 0, gs, s); }
             { Timed tm(T, "shade_mlt", s);
               bnd::launch_shade_mlt(fresh, sc.dev, P, M, ctl, (const uint32_t*)c->bins.p, bin_cap,
                                     (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p,
@@ -1251,6 +1252,10 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
             if (value != 0 && (value < 64 || value > ((int64_t)1 << 31)))
                 throw Error(BN_E_INVALID, "window_paths out of range");
             c->window_paths = (size_t)value;
+        } else if (k == "material_sort") {
+            if (value < -1 || value > 1) throw Error(BN_E_INVALID, "material_sort must be -1, 0 or 1");
+            c->material_sort = (int)value;
+            c->graphs.clear();
         } else if (k == "stack_k") {
             if (value != 16 && value != 32) throw Error(BN_E_INVALID, "stack_k must be 16 or 32");
             c->stack_k = (int)value;

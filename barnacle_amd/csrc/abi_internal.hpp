@@ -99,6 +99,7 @@ struct Ctx {
     bool use_graph = true;
     int stack_k = 16;                      // LDS traversal stack window
     int batch_pack = 0;                    // 0 item-major, 1 fair share
+    int material_sort = -1;                // bin shading by material kind; -1: adjoint rounds only
     DevBuf b_scenes, b_items, b_hdr;
     std::map<std::string, GraphExec> graphs;   // captured bounce loops by configuration
     // PSSMLT chain state

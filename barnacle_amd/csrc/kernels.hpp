@@ -106,6 +106,7 @@ struct Buffers {
     float* film;
     float4* L_out;            // radiance per path id of the round
     uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters (closest, shadow)
+    int sort;                 // bin the shade queue by material kind (1), not (0), adjoint only (-1)
     AdjBuf adj;               // used by Integrator::Adjoint only
 };
 
@@ -133,7 +134,7 @@ void enqueue_round_end(bool batch, Integrator kind, const Launch& L, const PathB
                        LaunchHook* hook);
 
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                uint32_t* freel, int grid, hipStream_t st);
+                uint32_t* freel, int sort, int grid, hipStream_t st);
 void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st);
 void launch_ctl_clear(Ctl* ctl, hipStream_t st);
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,

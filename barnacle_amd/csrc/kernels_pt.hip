@@ -327,17 +327,20 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BAT
     }
 }
 
-// Material binning of the closest hits (sorting the shade queue by material
-// kind).  A block bins kBinItems consecutive queue entries: positions inside
-// the block come from wave-aggregated LDS counters, and each bin costs one
-// global atomic per block.  A miss ends the path: with a free list (freel
-// non-null) its slot goes there, as one more bin; the PSSMLT driver passes
-// no list and retires misses itself.
+// Compaction of the closest hits for shading, optionally binned by material
+// kind (sort != 0).  Sorting makes waves shade one BSDF kind, but scatters
+// the slots each wave touches; on the mesh scenes measured here keeping the
+// queue order (ascending slots, coalesced path-state access) is faster, so it
+// is the default.  A block handles kBinItems consecutive queue entries:
+// positions inside the block come from wave-aggregated LDS counters, and each
+// bin costs one global atomic per block.  A miss ends the path: with a free
+// list (freel non-null) its slot goes there, as one more bin; the PSSMLT
+// driver passes no list and retires misses itself.
 constexpr int kBinPer = 16;
 constexpr int kBinItems = 256 * kBinPer;
 
 __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins,
-                                             uint32_t bin_cap, uint32_t* freel) {
+                                             uint32_t bin_cap, uint32_t* freel, int sort) {
     __shared__ uint32_t cnt[kBins + 1], gbase[kBins + 1];
     const uint32_t count = ctl->q_count[cur];
     const int n_out = freel ? kBins + 1 : kBins;
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
             if (i < count) {
                 slot[k] = queue[i];
                 const int b = P.hit[slot[k]].w;
-                bin[k] = b >= 0 ? b : (freel ? kBins : -1);
+                bin[k] = b >= 0 ? (sort ? b : 0) : (freel ? kBins : -1);
             }
             pos[k] = 0;
             for (int b = 0; b < n_out; ++b) {
@@ -925,7 +928,7 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
           launch_closest_t<BATCH>(count, B.stack_k, W, P, ctl, cur, B.queue[cur], B, L.refill, L.grid_trace, st); }
         { Bracket b(hook, "bin", st);
           hipLaunchKernelGGL(k_bin, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, cur, B.queue[cur], B.bins,
-                             B.bin_cap, B.freel[nxt]); }
+                             B.bin_cap, B.freel[nxt], B.sort < 0 ? kind == Integrator::Adjoint : B.sort); }
         if (kind == Integrator::Normal) {
             if constexpr (!BATCH) {
                 Bracket b(hook, "shade_normal", st);
@@ -991,8 +994,8 @@ void enqueue_round_end(bool batch, Integrator kind, const Launch& L, const PathB
 
 // Single kernels used by the PSSMLT driver (plain scene).
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
-                uint32_t* freel, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap, freel);
+                uint32_t* freel, int sort, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap, freel, sort);
 }
 void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(2 * kFetchSegs), 0, st, ctl, nxt, fetch_seg);

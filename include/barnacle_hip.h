@@ -297,7 +297,11 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
  *   "stack_k"              LDS window of the traversal stack: 16 (default) or 32
  *   "graph"                1 (default): replay batched rounds as a HIP graph
- *   "batch_pack"           0 (default): pack rounds item-major; 1: fair share */
+ *   "batch_pack"           0 (default): pack rounds item-major; 1: fair share
+ *   "material_sort"        1: bin the shading queue by material kind; 0: queue
+ *                          order, which keeps path-state access coalesced;
+ *                          -1 (default): bins for gradient rounds only, whose
+ *                          atomics gain from material-coherent waves */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {

@@ -19,6 +19,8 @@ SETTINGS = [
     {"stack_k": 32},
     {"stack_k": 32, "trace_blocks_per_cu": 2, "refill": 16},
     {"stack_k": 16, "trace_blocks_per_cu": 5, "refill": 64, "chunk_slots": 1000},
+    {"material_sort": 1},
+    {"material_sort": 0},
 ]
 
 
@@ -32,7 +34,8 @@ def test_launch_parameters_do_not_change_film(ctx, mesh_job, opts):
         got = ds.render_pt(p)
         films, _ = ctx.render_batch([(ds, p), (ds, p)])
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20}.items():
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20,
+                     "material_sort": -1}.items():
             ctx.set_option(k, v)
     assert np.array_equal(got.view(np.int32), ref.view(np.int32))
     for f in films:
@@ -53,5 +56,6 @@ def test_autotune(ctx, mesh_job):
         rb = ctx.autotune([(ds, p), (ds, p)], batched=True)
         assert rb["trials"] >= 8
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20}.items():
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 48, "chunk_slots": 64 << 20,
+                     "material_sort": -1}.items():
             ctx.set_option(k, v)
