@@ -102,6 +102,19 @@ struct Timed {
     ~Timed() { t.end(s); }
 };
 
+// True when the row-major 4x4 map has 1.0f on the diagonal and +0.0f (the bit
+// pattern 0) everywhere else.  Vector3.Transform by such a map returns
+// x*1 + y*0 + z*0 + 0 = x + 0.0f per coordinate: the input with -0 made +0.
+static bool is_identity_bits(const float* m) {
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            uint32_t u;
+            std::memcpy(&u, &m[4 * r + k], 4);
+            if (u != (r == k ? 0x3f800000u : 0u)) return false;
+        }
+    return true;
+}
+
 static bnd::Xf to_xf(const float* m) {
     bnd::Xf x;
     x.c0 = make_float4(m[0], m[4], m[8], m[12]);
@@ -218,6 +231,7 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
         d.hi = make_float4(in.bounds[3], in.bounds[4], in.bounds[5], pdf_const);
         d.info = make_int4(in.blas >= 0 ? node_base[in.blas] : -1, in.blas >= 0 ? tri_base[in.blas] : 0,
                            in.material, in.light);
+        d.aux = make_int4(is_identity_bits(in.w2o) ? 1 : 0, 0, 0, 0);
         if (in.light >= 0) lrec.push_back(make_int4((int)i, d.info.y, ntri, in.light));
     }
     for (const auto& m : mats)
@@ -804,7 +818,7 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
         if ((size_t)v >= slots_total) break;          // larger chunks change nothing
     }
     if (chunks.empty()) chunks.push_back(chunk);
-    std::vector<Knob> knobs = {{&blocks, {3, 4, 5, 6, 8}}, {&refill, {32, 40, 48, 56}}, {&stack_k, {16, 32}},
+    std::vector<Knob> knobs = {{&blocks, {3, 4, 5, 6, 8}}, {&refill, {24, 32, 40, 48}}, {&stack_k, {16, 32}},
                                {&chunk, chunks}};
     apply();
     double best = measure();

@@ -75,6 +75,7 @@ __device__ __forceinline__ float comp(V3 a, int i) { return i == 0 ? a.x : (i ==
 __device__ __forceinline__ V3 vmin(V3 a, V3 b) { return {a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z}; }
 __device__ __forceinline__ V3 vmax(V3 a, V3 b) { return {a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z}; }
 __device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ V3 sel(bool c, V3 a, V3 b) { return {c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
 // Affine map stored as three columns (M1j, M2j, M3j, M4j): Vector3.Transform.
@@ -117,6 +118,7 @@ struct DevInst {
     float4 lo;      // world bounds min, w = sphere radius
     float4 hi;      // world bounds max, w = EvalPDF constant of a mesh light
     int4 info;      // x = BLAS root node (-1: sphere), y = first triangle, z = material, w = light
+    int4 aux;       // x = 1: w2o is exactly the identity (entries +0.0f and 1.0f), yzw unused
 };
 struct DevScene {
     const float4* tlas;

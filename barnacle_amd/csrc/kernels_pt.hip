@@ -72,6 +72,10 @@ struct Work<false> {
     __device__ __forceinline__ const RoundItem& item(int) const { return it; }
     __device__ __forceinline__ const DevScene& scene(int) const { return S; }
     __device__ __forceinline__ uint32_t total_paths() const { return (uint32_t)it.path_count; }
+    // One scene for the whole launch: traversals bind it once, outside the
+    // divergent refill code, so its pointers stay wave-uniform.
+    __device__ __forceinline__ void bind(TraceScene& g) const { g.set(S); }
+    __device__ __forceinline__ void bind_lane(TraceScene&, int) const {}
 };
 
 template <>
@@ -93,6 +97,8 @@ struct Work<true> {
     __device__ __forceinline__ const RoundItem& item(int k) const { return items[k]; }
     __device__ __forceinline__ const DevScene& scene(int k) const { return scenes[items[k].scene]; }
     __device__ __forceinline__ uint32_t total_paths() const { return hdr->n_paths; }
+    __device__ __forceinline__ void bind(TraceScene&) const {}
+    __device__ __forceinline__ void bind_lane(TraceScene& g, int k) const { g.set(scene(k)); }
 };
 
 // local path id -> (pixel, sample): 8x8 pixel tiles, sample-major.
@@ -233,13 +239,15 @@ struct SegFetch {
     int segs;
 };
 
-template <bool ANY, bool COUNT, int K, typename Init, typename Done>
+template <bool ANY, bool COUNT, int K, typename Setup, typename Init, typename Done>
 __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uint32_t* stk, uint32_t* spill,
-                                                 size_t spill_stride, uint32_t refill, TraceCount& tc, Init init,
-                                                 Done done) {
+                                                 size_t spill_stride, uint32_t refill, TraceCount& tc, Setup setup,
+                                                 Init init, Done done) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     Traversal<K> tr;
     tr.st = LaneStack<K>{stk + threadIdx.x, (int)blockDim.x, spill + gid, spill_stride, 0};
+    tr.clear();
+    setup(tr);
     const uint32_t seg_len = (count + (uint32_t)F.segs - 1) / (uint32_t)F.segs;
     int seg = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (uint32_t)F.segs);
     int tried = 0;
@@ -278,11 +286,17 @@ __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uin
             if (!more) break;
             continue;
         }
-        bool fin = false, found = false;
-        if (active) fin = COUNT ? !tr.template step<ANY, COUNT>(tc, found) : !tr.template step_fast<ANY>(found);
+        // Every lane steps: an idle lane's cleared state has no item, so its
+        // step does nothing (and keeps the loop free of a per-lane branch).
+        bool found = false;
+        const bool going = COUNT ? tr.template step<ANY, COUNT>(tc, found) : tr.template step_fast<ANY>(found);
+        const bool fin = active && !going;
         if (__ballot(fin) != 0) {
-            if (fin) active = false;
             done(tr, item, fin, found);
+            if (fin) active = false;
+            // a finished closest-hit traversal is already empty; an any-hit
+            // one may stop with items left
+            if (ANY && found) tr.clear();
         }
     }
 }
@@ -301,10 +315,13 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BAT
     TraceCount tc{0, 0, 0};
     persistent_trace<false, COUNT, K>(
         SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = queue[i];
             const float4 o4 = P.o[slot];
-            tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.d[slot])}, kInf);
+            const int k = W.item_of(P, slot);
+            W.bind_lane(tr.G, k);
+            tr.start({xyz(o4), xyz(P.d[slot])}, kInf, W.scene(k).exact_only != 0);
             // a placeholder path (padding pixel, max-depth 0) misses at once
             if (__float_as_int(o4.w) < 0) tr.empty();
         },
@@ -514,10 +531,13 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATC
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT, K>(
         SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
-            tr.start(W.scene(W.item_of(P, slot)), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
+            const int k = W.item_of(P, slot);
+            W.bind_lane(tr.G, k);
+            tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, W.scene(k).exact_only != 0);
             // a NaN t max fails every slab test of the reference: empty traversal
             if (!(o4.w == o4.w)) tr.empty();
         },
@@ -716,10 +736,11 @@ __global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf
     TraceCount tc{0, 0, 0};
     persistent_trace<true, false, K>(
         SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
-            tr.start(W.scene(0), {xyz(o4), xyz(P.sh_d[slot])}, o4.w);
+            tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, W.S.exact_only != 0);
             if (!(o4.w == o4.w)) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
@@ -840,13 +861,14 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
     TraceCount tc{0, 0, 0};
     persistent_trace<false, false, kStackK>(
         SegFetch{fetch, 1}, n, stk, spill, spill_stride, 16u, tc,
+        [&](Traversal<kStackK>& tr) { tr.G.set(S); },
         [&](Traversal<kStackK>& tr, uint32_t p) {
             const int x = (int)(p % (uint32_t)width), y = (int)(p / (uint32_t)width);
             uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(frame_id * spp + sample_id));
             V2 uPix, uLens;
             uPix.x = lcg(rng); uPix.y = lcg(rng);
             uLens.x = lcg(rng); uLens.y = lcg(rng);
-            tr.start(S, camera_ray(S, x, y, uPix, uLens), kInf);
+            tr.start(camera_ray(S, x, y, uPix, uLens), kInf, S.exact_only != 0);
         },
         [&](Traversal<kStackK>& tr, uint32_t p, bool fin, bool) {
             if (fin) {

@@ -31,6 +31,19 @@ __device__ __forceinline__ bool slab_fast(V3 lo, V3 hi, V3 o, V3 inv, float t) {
     const float tmax = fminf(fminf(t, fmaxf(t0x, t1x)), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
     return tmin <= tmax;
 }
+// Three-way min / max in one instruction.  Used only on the finite path, where
+// they agree with the compare-and-select Vector3.Min/Max up to the sign of a
+// zero, which no slab comparison can see.
+__device__ __forceinline__ float min3f(float a, float b, float c) {
+    float r;
+    asm volatile("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float max3f(float a, float b, float c) {
+    float r;
+    asm volatile("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 __device__ __forceinline__ bool finite3(V3 v) {
     return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
@@ -66,9 +79,9 @@ struct Traversal {
     int bl_cur, bl_end;  // triangles left in the current BLAS leaf (global indices)
     int blas_root, tri_base, cur_inst;
 
-    __device__ __forceinline__ void start(const DevScene& S, const Ray& r, float tmax) {
-        G.set(S);
-        const bool exact_scene = S.exact_only != 0;
+    // A new ray over the geometry already in G (set once per launch for a
+    // single scene, per ray in a batch).
+    __device__ __forceinline__ void start(const Ray& r, float tmax, bool exact_scene) {
         wo = r.o; wd = r.d;
         winv = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
         co = wo; cd = wd; cinv = winv;
@@ -85,9 +98,33 @@ struct Traversal {
 
     // No traversal at all (placeholder rays, NaN t max).
     __device__ __forceinline__ void empty() { st.sp = 0; nxt = kNoNode; }
+    // An idle lane: no item left anywhere, so next_kind() reports kItemNone
+    // and a step does nothing.
+    __device__ __forceinline__ void clear() {
+        st.sp = 0;
+        nxt = kNoNode;
+        blas_base = -1;
+        tl_cur = tl_end = 0;
+        bl_cur = bl_end = 0;
+    }
 
+    // The fast test is evaluated first, in the block that computes its
+    // products, so its min/max need no operand canonicalisation; the exact
+    // test replaces it on the rare non-finite rays.
     __device__ __forceinline__ bool box(V3 lo, V3 hi) const {
-        return exact ? slab(lo, hi, co, cinv, t) : slab_fast(lo, hi, co, cinv, t);
+        bool r = slab_fast(lo, hi, co, cinv, t);
+        if (exact) r = slab(lo, hi, co, cinv, t);
+        return r;
+    }
+    // A triangle's bounding-box pre-test (Mesh.fs:204).
+    __device__ __forceinline__ bool box_tri(const Tri& tr) const {
+        if (!exact)
+            return slab_fast(mk(min3f(tr.p0.x, tr.p1.x, tr.p2.x), min3f(tr.p0.y, tr.p1.y, tr.p2.y),
+                                min3f(tr.p0.z, tr.p1.z, tr.p2.z)),
+                             mk(max3f(tr.p0.x, tr.p1.x, tr.p2.x), max3f(tr.p0.y, tr.p1.y, tr.p2.y),
+                                max3f(tr.p0.z, tr.p1.z, tr.p2.z)),
+                             co, cinv, t);
+        return slab(vmin(vmin(tr.p0, tr.p1), tr.p2), vmax(vmax(tr.p0, tr.p1), tr.p2), co, cinv, t);
     }
 
     // One item.  Returns false when the traversal has finished (no items left,
@@ -192,10 +229,13 @@ struct Traversal {
     // What this lane does next (leaving a finished BLAS first): kItemTri,
     // kItemInst, kItemTNode (any node), or kItemNone when it has finished.
     __device__ __forceinline__ int next_kind() {
-        if (blas_base >= 0 && bl_cur >= bl_end && nxt == kNoNode && st.sp == blas_base) {
-            blas_base = -1;
-            co = wo; cd = wd; cinv = winv;
-        }
+        // Selects rather than a branch: the ray registers are then updated in
+        // place instead of through copies at the join.
+        const bool leave = blas_base >= 0 && bl_cur >= bl_end && nxt == kNoNode && st.sp == blas_base;
+        blas_base = leave ? -1 : blas_base;
+        co = sel(leave, wo, co);
+        cd = sel(leave, wd, cd);
+        cinv = sel(leave, winv, cinv);
         if (blas_base >= 0 && bl_cur < bl_end) return kItemTri;
         if (blas_base < 0 && tl_cur < tl_end) return kItemInst;
         if (nxt != kNoNode || st.sp > (blas_base >= 0 ? blas_base : 0)) return kItemTNode;
@@ -207,7 +247,7 @@ struct Traversal {
     __device__ __forceinline__ bool do_tri(bool& found) {
         const int idx = bl_cur++;
         const Tri tr = load_tri(G.tris, idx);
-        if (!box(vmin(vmin(tr.p0, tr.p1), tr.p2), vmax(vmax(tr.p0, tr.p1), tr.p2))) return true;
+        if (!box_tri(tr)) return true;
         float tt;
         if (tri_hit(tr, co, cd, t, tt)) {
             if (ANY) { found = true; return false; }
@@ -225,9 +265,23 @@ struct Traversal {
         const int idx = tl_cur++;
         const DevInst& I = G.insts[idx];
         if (!box(xyz(I.lo), xyz(I.hi))) return true;
+        const int4 info = I.info;
+        if (info.x >= 0 && I.aux.x != 0 && !exact) {
+            // Identity world-to-object map on a finite ray: Transform gives
+            // o + 0 per coordinate and d unchanged (no component of d is 0
+            // when 1/d is finite), so the object ray's 1/d is the world one.
+            co = mk(wo.x + 0.0f, wo.y + 0.0f, wo.z + 0.0f);
+            cd = wd;
+            cinv = winv;
+            cur_inst = idx;
+            blas_root = info.x;
+            tri_base = info.y;
+            blas_base = st.sp;
+            nxt = kNeedTest;
+            return true;
+        }
         const Xf w2o = I.w2o;
         const V3 oo = xf_point(w2o, wo), od = xf_dir(w2o, wd);
-        const int4 info = I.info;
         if (info.x < 0) {
             float tt;
             int br;
