@@ -309,9 +309,13 @@ void ensure_paths(Ctx* c, size_t cap) {
     if (cap <= c->path_cap) return;
     auto f4 = [&](DevBuf& b) { b.ensure(cap * sizeof(float4)); };
     f4(c->p_o); f4(c->p_d); f4(c->p_beta); f4(c->p_L); f4(c->p_sh_o); f4(c->p_sh_d); f4(c->p_nee_a); f4(c->p_nee_b);
+    f4(c->p2_o); f4(c->p2_d); f4(c->p2_beta); f4(c->p2_L);
     c->p_rng.ensure(cap * 4);
     c->p_item.ensure(cap * 4);
     c->p_pid.ensure(cap * 4);
+    c->p2_rng.ensure(cap * 4);
+    c->p2_item.ensure(cap * 4);
+    c->p2_pid.ensure(cap * 4);
     c->p_hit.ensure(cap * sizeof(int4));
     c->queue[0].ensure(cap * 4);
     c->queue[1].ensure(cap * 4);
@@ -360,6 +364,8 @@ bnd::Buffers buffers(Ctx* c) {
     c->fetch_seg.ensure((size_t)2 * bnd::kFetchSegs * bnd::kSegStride * 4);
     B.fetch_seg = (uint32_t*)c->fetch_seg.p;
     B.sort = c->material_sort;
+    B.alt = bnd::PathState{(float4*)c->p2_o.p, (float4*)c->p2_d.p, (float4*)c->p2_beta.p, (float4*)c->p2_L.p,
+                           (uint32_t*)c->p2_rng.p, (int*)c->p2_item.p, (uint32_t*)c->p2_pid.p};
     return B;
 }
 
@@ -786,8 +792,8 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
     }
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    // path state + queues + bins per slot
-    const size_t per_slot = 8 * sizeof(float4) + sizeof(int4) + 4 * 8 + 4 * bnd::kBins;
+    // path state (two copies of the moving part) + queues + bins per slot
+    const size_t per_slot = 12 * sizeof(float4) + sizeof(int4) + 4 * 11 + 4 * bnd::kBins;
     const size_t slot_limit = (free_b + c->path_cap * per_slot) / 4 / per_slot;
 
     auto run = [&]() -> double {

@@ -85,6 +85,7 @@ struct Ctx {
     // path state
     size_t path_cap = 0;
     DevBuf p_o, p_d, p_beta, p_L, p_rng, p_hit, p_sh_o, p_sh_d, p_nee_a, p_nee_b, p_item, p_pid;
+    DevBuf p2_o, p2_d, p2_beta, p2_L, p2_rng, p2_item, p2_pid;   // second state copy (compact rounds)
     DevBuf queue[2], freel[2], shq, bins, spill, film, ctl, stats, L_out, fetch_seg;
     // path-replay adjoint: per-slot terms, gradients, film adjoint
     DevBuf a_delta, a_pend_a, a_pend_r, a_nee_da, a_nee_dr, a_gmat, a_glight, a_dfilm;

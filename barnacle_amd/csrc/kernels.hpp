@@ -33,6 +33,22 @@ struct PathBuf {
     uint32_t* pid;    // path id of the slot within its round, kNoPath if idle
 };
 
+// The second copy of the per-path state of compact rounds.  There the active
+// paths of an iteration are dense in positions [0, count) of one copy, and
+// shading writes the paths that continue, compacted, into the other copy
+// (hit, shadow-ray and NEE arrays are shared and indexed by position).
+struct PathState {
+    float4 *o, *d, *beta, *L;
+    uint32_t* rng;
+    int* item;
+    uint32_t* pid;
+};
+
+// Target of a compact round's shadow ray (in nee_a.w): the position of its
+// path in the next state copy, or kTargetDone | path id when the path has
+// ended and its radiance already sits in the per-id output.
+constexpr uint32_t kTargetDone = 0x80000000u;
+
 // Device-side counters of the iterations of a round.
 struct Ctl {
     uint32_t q_count[2];      // active queues (this iteration, next iteration)
@@ -108,6 +124,7 @@ struct Buffers {
     uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters (closest, shadow)
     int sort;                 // bin the shade queue by material kind (1), not (0), adjoint only (-1)
     AdjBuf adj;               // used by Integrator::Adjoint only
+    PathState alt;            // second state copy: Integrator::Path rounds run compact when it is set
 };
 
 // Called around every launch of a round (timing, counters).
@@ -125,6 +142,11 @@ struct LaunchHook {
 //   iterations: n (even) x [regen, prep, trace_closest, bin, shade, trace_shadow]
 //   end:        retire the last paths, fold the round into the films (not for
 //               adjoint rounds, which only accumulate gradients)
+// Path-tracing rounds run compact when a second state copy is given (and
+// material binning is not forced): no free lists, queues or bins; regen
+// appends new paths after the continuing ones, shading moves the continuing
+// paths to the other copy and retires the ended ones by path id.
+//   iterations: n (even) x [regen, prep, trace_closest, shade, trace_shadow]
 enum class Integrator { Path, Normal, Adjoint };
 void enqueue_round_begin(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
                          LaunchHook* hook);

@@ -186,16 +186,55 @@ __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const C
     }
 }
 
+// Compact rounds: the new paths take positions after the n_act paths that
+// continue from the previous iteration, up to n_slots, with the seeding and
+// primary rays of k_regen.  Ended paths were retired by the shading kernels.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_regen_c(Work<BATCH> W, PathBuf X, const Ctl* ctl, int cur,
+                                                 uint32_t n_slots) {
+    const uint32_t n_act = ctl->q_count[cur];
+    const uint32_t next_pid = ctl->next_pid;
+    const uint32_t n_new = new_paths(n_slots - n_act, next_pid, W.total_paths());
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += gridDim.x * blockDim.x) {
+        const uint32_t pos = n_act + j;
+        const uint32_t id = next_pid + j;
+        const int k = W.item_of_path(id);
+        const RoundItem& it = W.item(k);
+        int x = 0, y = 0, s = 0;
+        if (path_pixel(it, id - (uint32_t)it.path_base, x, y, s) && it.max_depth > 0) {
+            const DevScene& S = W.scene(k);
+            uint32_t rng = xxhash3((uint32_t)x, (uint32_t)y, (uint32_t)(it.frame_id * it.spp + s));
+            V2 uPix, uLens;
+            uPix.x = lcg(rng); uPix.y = lcg(rng);
+            uLens.x = lcg(rng); uLens.y = lcg(rng);
+            const Ray r = camera_ray(S, x, y, uPix, uLens);
+            X.o[pos] = f4(r.o, __int_as_float(0));
+            X.d[pos] = f4(r.d, 0.0f);
+            X.beta[pos] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            X.rng[pos] = rng;
+        } else {
+            X.o[pos] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+            X.d[pos] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        }
+        X.L[pos] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        X.pid[pos] = id;
+        if (BATCH) X.item[pos] = k;
+    }
+}
+
 // After regen: the new paths join the active queue, the ids advance, and the
 // counters of the other parity (filled during this iteration) start at zero.
+// compact_slots: 0 for slot rounds (new paths fill the free list), else the
+// capacity of a compact round.
 template <bool BATCH>
-__global__ void k_prep(Work<BATCH> W, Ctl* ctl, int cur, uint32_t* fetch_seg) {
+__global__ void k_prep(Work<BATCH> W, Ctl* ctl, int cur, uint32_t* fetch_seg, uint32_t compact_slots) {
     const int i = threadIdx.x;
     const int nxt = 1 - cur;
     if (i < 2 * kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
     if (i < kBins) ctl->bin_count[i] = 0;
     if (i == 0) {
-        const uint32_t n_new = new_paths(ctl->free_count[cur], ctl->next_pid, W.total_paths());
+        const uint32_t n_free = compact_slots ? compact_slots - ctl->q_count[cur] : ctl->free_count[cur];
+        const uint32_t n_new = new_paths(n_free, ctl->next_pid, W.total_paths());
         ctl->q_count[cur] += n_new;
         ctl->next_pid += n_new;
         ctl->sh_count = 0;
@@ -303,9 +342,10 @@ __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uin
 
 // BVHAggregate.Intersect closest (Extensions/Aggregate/BVH.fs:37-58).  The
 // material bin of the hit (kind of the instance's material; light-only
-// instances in the last bin, -1 for a miss) travels in the hit record.
+// instances in the last bin, -1 for a miss) travels in the hit record.  The
+// rays are queue[i], or position i of a compact round (queue null).
 // With a 16-entry LDS stack window six blocks fit a CU's LDS; the register
-// budget is then set for six waves per SIMD (a little spilling included).
+// budget is then set for six waves per SIMD.
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* spill, size_t spill_stride,
@@ -317,7 +357,7 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BAT
         SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
-            const uint32_t slot = queue[i];
+            const uint32_t slot = queue ? queue[i] : i;
             const float4 o4 = P.o[slot];
             const int k = W.item_of(P, slot);
             W.bind_lane(tr.G, k);
@@ -327,7 +367,7 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BAT
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool) {
             if (fin) {
-                const uint32_t slot = queue[i];
+                const uint32_t slot = queue ? queue[i] : i;
                 int bin = -1;
                 if (tr.hit_inst >= 0) {
                     const DevScene& S = W.scene(W.item_of(P, slot));
@@ -401,11 +441,11 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
     }
 }
 
-// Block-aggregated append to N queues: one global atomic per queue and block.
+// Block-aggregated positions in N queues: one global atomic per queue and
+// block.  pos[o] is the thread's position in queue o when pred[o] holds.
 // Every thread of the (256-thread) block must call it.
 template <int N>
-__device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* const (&ctr)[N],
-                                           uint32_t* const (&arr)[N], uint32_t val) {
+__device__ __forceinline__ void block_pos(const bool (&pred)[N], uint32_t* const (&ctr)[N], uint32_t (&pos)[N]) {
     __shared__ uint32_t wcnt[N][5];
     const int w = (int)(threadIdx.x >> 6);
     uint64_t m[N];
@@ -425,18 +465,98 @@ __device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* cons
         wcnt[o][4] = run ? atomicAdd(ctr[o], run) : 0u;
     }
     __syncthreads();
-    for (int o = 0; o < N; ++o)
-        if (pred[o]) arr[o][wcnt[o][4] + wcnt[o][w] + mbcnt(m[o])] = val;
+    for (int o = 0; o < N; ++o) pos[o] = wcnt[o][4] + wcnt[o][w] + mbcnt(m[o]);
     __syncthreads();
 }
 
+// Block-aggregated append of `val` to N queues.
+template <int N>
+__device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* const (&ctr)[N],
+                                           uint32_t* const (&arr)[N], uint32_t val) {
+    uint32_t pos[N];
+    block_pos<N>(pred, ctr, pos);
+    for (int o = 0; o < N; ++o)
+        if (pred[o]) arr[o][pos[o]] = val;
+}
+
 // ------------------------------------------------------------------- shade
-// The body of one loop iteration of PathTracing.fs:24-79 after the closest
-// hit: emission with MIS (:30-40), next-event estimation set-up (:42-50), BSDF
-// sampling (:61-67) and Russian roulette (:69-75).  The NEE contribution is
-// applied by k_trace_shadow when the shadow ray is unoccluded (:47-59).
-// The path's bounce depth is o.w; a path that does not continue frees its
-// slot for the next iteration (its NEE term still lands in this iteration).
+// One vertex of PathTracing.fs:24-79 after the closest hit: emission with MIS
+// (:30-40), next-event estimation set-up (:42-50), BSDF sampling (:61-67) and
+// Russian roulette (:69-75).  The NEE contribution is applied by
+// k_trace_shadow when the shadow ray is unoccluded (:47-59).  The path's
+// bounce depth is o.w.  The state is read at index `at` of P; the results
+// are returned for the caller to store.
+struct Vertex {
+    V3 L;                             // radiance after this vertex's emission term
+    bool cont, shadow;                // the path continues / casts a shadow ray
+    float4 o, d, beta;                // next ray (o.w depth, d.w BSDF pdf) and throughput, when cont
+    uint32_t rng;
+    float4 sh_o, sh_d, nee_a, nee_b;  // shadow ray (sh_o.w t max) and NEE factors, when shadow
+};
+__device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem& it, const PathBuf& P, uint32_t at,
+                                             int4 h, Vertex& v) {
+    const float4 o4 = P.o[at], d4 = P.d[at];
+    const Ray r = {xyz(o4), xyz(d4)};
+    const float prev_pdf = d4.w;
+    const int depth = __float_as_int(o4.w);
+    const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+    const int4 info = S.insts[h.x].info;
+    V3 L = xyz(P.L[at]);
+    V3 beta = xyz(P.beta[at]);
+    v.cont = false;
+    v.shadow = false;
+    if (info.w >= 0) {
+        V3 Le;
+        float lpdf;
+        light_eval(S, h.x, r.o, g, Le, lpdf);
+        const float w = depth == 0 ? 1.0f : prev_pdf * rcp_e(lpdf + prev_pdf);
+        L = fma3(beta, Le * w, L);
+    }
+    if (info.z >= 0) {
+        uint32_t rng = P.rng[at];
+        const Mat m = load_mat(S, info.z);
+        const V3 wo = -r.d;
+        const V3 wo_l = to_local(g, wo);
+        const float uSel = lcg(rng);
+        V2 uL;
+        uL.x = lcg(rng); uL.y = lcg(rng);
+        const LightSample ls = light_sample(S, g.p, uSel, uL);
+        if (ls.pdf != 0.0f) {
+            const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
+            const float dist = length(ls.p - g.p);
+            v.sh_o = f4(g.p, dist - 1e-3f);
+            v.sh_d = f4(ls.wi, 0.0f);
+            v.nee_a = f4(beta * e.f, 0.0f);
+            v.nee_b = f4(ls.L * rcp_e(e.pdf + ls.pdf), 0.0f);
+            v.shadow = true;
+        }
+        const float ul = lcg(rng);
+        V2 ub;
+        ub.x = lcg(rng); ub.y = lcg(rng);
+        V3 wi_l;
+        const BSDF bs = mat_sample(m, wo_l, ul, ub, wi_l);
+        if (bs.pdf != 0.0f) {
+            const V3 wi = to_world(g, wi_l);
+            beta = beta * bs.f * rcp_e(bs.pdf);
+            v.cont = true;
+            if (depth >= it.rr_depth) {
+                const float q = min_n(1.0f, max_n(beta.x, max_n(beta.y, beta.z)));
+                if (lcg(rng) < q) beta = beta * rcp_e(q);
+                else v.cont = false;
+            }
+            v.o = f4(g.p, __int_as_float(depth + 1));
+            v.d = f4(wi, bs.pdf);
+            v.beta = f4(beta, 0.0f);
+        }
+        v.rng = rng;
+        v.cont = v.cont && depth + 1 < it.max_depth;
+    }
+    v.L = L;
+}
+
+// Slot rounds: the vertices of the bins, shaded in place; a path that does
+// not continue frees its slot for the next iteration (its NEE term still
+// lands in this iteration).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* bins,
                                                uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
@@ -447,83 +567,101 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
     const uint32_t total = start[kBins];
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, shadow = false, fin = false;
+        Vertex v;
+        v.cont = false;
+        v.shadow = false;
         uint32_t slot = 0;
         if (i < total) {
             int b = 0;
             while (i >= start[b + 1]) ++b;
             slot = bins[(size_t)b * bin_cap + (i - start[b])];
             const int k = W.item_of(P, slot);
-            const DevScene& S = W.scene(k);
-            const RoundItem& it = W.item(k);
-            const float4 o4 = P.o[slot], d4 = P.d[slot];
-            const Ray r = {xyz(o4), xyz(d4)};
-            const float prev_pdf = d4.w;
-            const int depth = __float_as_int(o4.w);
-            const int4 h = P.hit[slot];
-            const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
-            const int4 info = S.insts[h.x].info;
-            V3 L = xyz(P.L[slot]);
-            V3 beta = xyz(P.beta[slot]);
-            if (info.w >= 0) {
-                V3 Le;
-                float lpdf;
-                light_eval(S, h.x, r.o, g, Le, lpdf);
-                const float w = depth == 0 ? 1.0f : prev_pdf * rcp_e(lpdf + prev_pdf);
-                L = fma3(beta, Le * w, L);
+            shade_vertex(W.scene(k), W.item(k), P, slot, P.hit[slot], v);
+            if (v.shadow) {
+                P.sh_o[slot] = v.sh_o;
+                P.sh_d[slot] = v.sh_d;
+                P.nee_a[slot] = v.nee_a;
+                P.nee_b[slot] = v.nee_b;
             }
-            if (info.z >= 0) {
-                uint32_t rng = P.rng[slot];
-                const Mat m = load_mat(S, info.z);
-                const V3 wo = -r.d;
-                const V3 wo_l = to_local(g, wo);
-                const float uSel = lcg(rng);
-                V2 uL;
-                uL.x = lcg(rng); uL.y = lcg(rng);
-                const LightSample ls = light_sample(S, g.p, uSel, uL);
-                if (ls.pdf != 0.0f) {
-                    const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
-                    const float dist = length(ls.p - g.p);
-                    P.sh_o[slot] = f4(g.p, dist - 1e-3f);
-                    P.sh_d[slot] = f4(ls.wi, 0.0f);
-                    P.nee_a[slot] = f4(beta * e.f, 0.0f);
-                    P.nee_b[slot] = f4(ls.L * rcp_e(e.pdf + ls.pdf), 0.0f);
-                    shadow = true;
-                }
-                const float ul = lcg(rng);
-                V2 ub;
-                ub.x = lcg(rng); ub.y = lcg(rng);
-                V3 wi_l;
-                const BSDF bs = mat_sample(m, wo_l, ul, ub, wi_l);
-                if (bs.pdf != 0.0f) {
-                    const V3 wi = to_world(g, wi_l);
-                    beta = beta * bs.f * rcp_e(bs.pdf);
-                    cont = true;
-                    if (depth >= it.rr_depth) {
-                        const float q = min_n(1.0f, max_n(beta.x, max_n(beta.y, beta.z)));
-                        if (lcg(rng) < q) beta = beta * rcp_e(q);
-                        else cont = false;
-                    }
-                    P.o[slot] = f4(g.p, __int_as_float(depth + 1));
-                    P.d[slot] = f4(wi, bs.pdf);
-                    P.beta[slot] = f4(beta, 0.0f);
-                }
-                P.rng[slot] = rng;
-                cont = cont && depth + 1 < it.max_depth;
+            if (v.cont) {
+                P.o[slot] = v.o;
+                P.d[slot] = v.d;
+                P.beta[slot] = v.beta;
+                P.rng[slot] = v.rng;
             }
-            P.L[slot] = f4(L, 0.0f);
-            fin = !cont;
+            P.L[slot] = f4(v.L, 0.0f);
         }
-        block_push<3>({cont, shadow, fin}, {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
-                      {queue_next, shq, freel}, slot);
+        block_push<3>({v.cont, v.shadow, i < total && !v.cont},
+                      {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]}, {queue_next, shq, freel}, slot);
     }
+}
+
+// Compact rounds: position i of the current copy X for every active path
+// (misses included: they end here).  A path that continues moves to its
+// compacted position in the other copy Y; one that ends leaves its radiance
+// in L_out[path id].  Shadow rays are stored by their own position, with the
+// item (batch) in sh_d.w and the target of the NEE term in nee_a.w.  The
+// vertices shaded are counted in bin_count[0].
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur, int nxt,
+                                                 float4* L_out) {
+    const uint32_t total = ctl->q_count[cur];
+    uint32_t shaded = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        Vertex v;
+        v.cont = false;
+        v.shadow = false;
+        uint32_t pid = 0;
+        int k = 0;
+        if (i < total) {
+            pid = X.pid[i];
+            k = W.item_of(X, i);
+            const int4 h = X.hit[i];
+            if (h.x >= 0) {
+                ++shaded;
+                shade_vertex(W.scene(k), W.item(k), X, i, h, v);
+            } else {
+                v.L = xyz(X.L[i]);
+            }
+        }
+        uint32_t pos[2];
+        block_pos<2>({v.cont, v.shadow}, {&ctl->q_count[nxt], &ctl->sh_count}, pos);
+        if (i < total) {
+            if (v.cont) {
+                const uint32_t j = pos[0];
+                Y.o[j] = v.o;
+                Y.d[j] = v.d;
+                Y.beta[j] = v.beta;
+                Y.L[j] = f4(v.L, 0.0f);
+                Y.rng[j] = v.rng;
+                Y.pid[j] = pid;
+                if (BATCH) Y.item[j] = k;
+            } else {
+                L_out[pid] = f4(v.L, 0.0f);
+            }
+            if (v.shadow) {
+                const uint32_t q = pos[1];
+                X.sh_o[q] = v.sh_o;
+                X.sh_d[q] = make_float4(v.sh_d.x, v.sh_d.y, v.sh_d.z, __int_as_float(k));
+                X.nee_a[q] = make_float4(v.nee_a.x, v.nee_a.y, v.nee_a.z,
+                                         __uint_as_float(v.cont ? pos[0] : (kTargetDone | pid)));
+                X.nee_b[q] = v.nee_b;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) shaded += __shfl_xor(shaded, off);
+    if (lane_id() == 0 && shaded) atomicAdd(&ctl->bin_count[0], shaded);
 }
 
 // ----------------------------------------------------------- trace shadow
 // aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
-// an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.
+// an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.  Slot rounds: the
+// rays of the slots in shq, added to the slot's L.  Compact rounds (shq
+// null): ray i, added to L_next[target] or to L_out[path id] (nee_a.w).
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
+                                                      float4* L_next, float4* L_out,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -533,19 +671,25 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATC
         SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
-            const uint32_t slot = shq[i];
-            const float4 o4 = P.sh_o[slot];
-            const int k = W.item_of(P, slot);
+            const uint32_t slot = shq ? shq[i] : i;
+            const float4 o4 = P.sh_o[slot], d4 = P.sh_d[slot];
+            const int k = shq ? W.item_of(P, slot) : (BATCH ? __float_as_int(d4.w) : 0);
             W.bind_lane(tr.G, k);
-            tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, W.scene(k).exact_only != 0);
+            tr.start({xyz(o4), xyz(d4)}, o4.w, W.scene(k).exact_only != 0);
             // a NaN t max fails every slab test of the reference: empty traversal
             if (!(o4.w == o4.w)) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
             if (fin && !found) {
-                const uint32_t slot = shq[i];
-                const V3 a = xyz(P.nee_a[slot]), b = xyz(P.nee_b[slot]);
-                P.L[slot] = f4(fma3(a, b, xyz(P.L[slot])), 0.0f);
+                const uint32_t slot = shq ? shq[i] : i;
+                const float4 a = P.nee_a[slot];
+                const V3 b = xyz(P.nee_b[slot]);
+                float4* dst = &P.L[slot];
+                if (!shq) {
+                    const uint32_t t = __float_as_uint(a.w);
+                    dst = (t & kTargetDone) ? &L_out[t & ~kTargetDone] : &L_next[t];
+                }
+                *dst = f4(fma3(xyz(a), b, xyz(*dst)), 0.0f);
             }
         });
     if (COUNT) {
@@ -907,10 +1051,11 @@ static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const Path
 }
 template <bool BATCH>
 static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                            const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
-#define BN_TS(C, KK)                                                                                              \
-    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, B.spill, \
-                       B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
+                            float4* L_next, float4* L_out, const Buffers& B, uint32_t refill, int grid,
+                            hipStream_t st) {
+#define BN_TS(C, KK)                                                                                            \
+    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_next,  \
+                       L_out, B.spill, B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
     if (k == 16) { if (count) BN_TS(true, 16); else BN_TS(false, 16); }
     else { if (count) BN_TS(true, 32); else BN_TS(false, 32); }
 #undef BN_TS
@@ -931,9 +1076,50 @@ static void round_begin_t(const Launch& L, const PathBuf& P, Ctl* ctl, const Buf
       hipLaunchKernelGGL(k_init_slots, dim3(L.grid_shade), dim3(256), 0, st, P, ctl, B.freel[0], L.n_slots); }
 }
 
+// Path-tracing rounds run compact unless material binning is forced.
+static bool compact_round(Integrator kind, const Buffers& B) {
+    return kind == Integrator::Path && B.alt.o != nullptr && B.sort != 1;
+}
+static PathBuf with_state(PathBuf P, const PathState& S) {
+    P.o = S.o; P.d = S.d; P.beta = S.beta; P.L = S.L;
+    P.rng = S.rng; P.item = S.item; P.pid = S.pid;
+    return P;
+}
+static PathState state_of(const PathBuf& P) { return PathState{P.o, P.d, P.beta, P.L, P.rng, P.item, P.pid}; }
+
+template <bool BATCH>
+static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const PathBuf& P, Ctl* ctl,
+                                 const Buffers& B, int n, bool count, LaunchHook* hook) {
+    hipStream_t st = L.stream;
+    const PathBuf copies[2] = {P, with_state(P, B.alt)};
+    for (int i = 0; i < n; ++i) {
+        const int cur = i & 1, nxt = 1 - cur;
+        const PathBuf& X = copies[cur];
+        const PathBuf& Y = copies[nxt];
+        { Bracket b(hook, "regen", st);
+          hipLaunchKernelGGL(k_regen_c<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, X, ctl, cur, L.n_slots); }
+        { Bracket b(hook, "prep", st);
+          hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg,
+                             L.n_slots); }
+        { Bracket b(hook, "trace_closest", st);
+          launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
+        { Bracket b(hook, "shade", st);
+          hipLaunchKernelGGL(k_shade_c<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, X, state_of(Y), ctl, cur,
+                             nxt, B.L_out); }
+        { Bracket b(hook, "trace_shadow", st);
+          launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, Y.L, B.L_out, B, L.refill, L.grid_trace,
+                                 st); }
+        if (hook) hook->after_bounce(cur);
+    }
+}
+
 template <bool BATCH>
 static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl,
                          const Buffers& B, int n, bool count, LaunchHook* hook) {
+    if (compact_round(kind, B)) {
+        compact_iterations_t<BATCH>(W, L, P, ctl, B, n, count, hook);
+        return;
+    }
     hipStream_t st = L.stream;
     for (int i = 0; i < n; ++i) {
         const int cur = i & 1, nxt = 1 - cur;
@@ -945,7 +1131,7 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
               hipLaunchKernelGGL((k_regen<BATCH, false>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, cur,
                                  B.freel[cur], B.queue[cur], B.L_out, B.adj); }
         { Bracket b(hook, "prep", st);
-          hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg); }
+          hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg, 0u); }
         { Bracket b(hook, "trace_closest", st);
           launch_closest_t<BATCH>(count, B.stack_k, W, P, ctl, cur, B.queue[cur], B, L.refill, L.grid_trace, st); }
         { Bracket b(hook, "bin", st);
@@ -978,7 +1164,8 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
               hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
                                  B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
             { Bracket b(hook, "trace_shadow", st);
-              launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, B, L.refill, L.grid_trace, st); }
+              launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, nullptr, nullptr, B, L.refill, L.grid_trace,
+                                     st); }
         }
         if (hook) hook->after_bounce(cur);
     }
@@ -988,11 +1175,14 @@ template <bool BATCH>
 static void round_end_t(const Work<BATCH>& W, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl,
                         const Buffers& B, LaunchHook* hook) {
     hipStream_t st = L.stream;
-    // iteration blocks are even: the last paths sit in free list 0
+    // iteration blocks are even: the last paths sit in free list 0 (a compact
+    // round has retired every path in its shading and shadow kernels)
     if (kind == Integrator::Adjoint) return;               // nothing to retire, no film
-    { Bracket b(hook, "regen", st);
-      hipLaunchKernelGGL((k_regen<BATCH, false>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, 0, B.freel[0],
-                         B.queue[0], B.L_out, B.adj); }
+    if (!compact_round(kind, B)) {
+        Bracket b(hook, "regen", st);
+        hipLaunchKernelGGL((k_regen<BATCH, false>), dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, 0, B.freel[0],
+                           B.queue[0], B.L_out, B.adj);
+    }
     { Bracket b(hook, "film", st);
       hipLaunchKernelGGL(k_film<BATCH>, dim3(BATCH ? L.grid_shade : (L.n_pix + 255) / 256), dim3(256), 0, st, W,
                          B.L_out, B.film); }
@@ -1037,7 +1227,7 @@ void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* c
     Work<false> W;
     W.S = S;
     W.it = RoundItem{};
-    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, B, refill, grid, st);
+    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, nullptr, nullptr, B, refill, grid, st);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {
