@@ -362,6 +362,7 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BAT
             const int k = W.item_of(P, slot);
             W.bind_lane(tr.G, k);
             tr.start({xyz(o4), xyz(P.d[slot])}, kInf, W.scene(k).exact_only != 0);
+            if (!COUNT) tr.take_tlas_root();
             // a placeholder path (padding pixel, max-depth 0) misses at once
             if (__float_as_int(o4.w) < 0) tr.empty();
         },
@@ -676,6 +677,7 @@ __global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATC
             const int k = shq ? W.item_of(P, slot) : (BATCH ? __float_as_int(d4.w) : 0);
             W.bind_lane(tr.G, k);
             tr.start({xyz(o4), xyz(d4)}, o4.w, W.scene(k).exact_only != 0);
+            if (!COUNT) tr.take_tlas_root();
             // a NaN t max fails every slab test of the reference: empty traversal
             if (!(o4.w == o4.w)) tr.empty();
         },
@@ -885,6 +887,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf
             const uint32_t slot = shq[i];
             const float4 o4 = P.sh_o[slot];
             tr.start({xyz(o4), xyz(P.sh_d[slot])}, o4.w, W.S.exact_only != 0);
+            tr.take_tlas_root();
             if (!(o4.w == o4.w)) tr.empty();
         },
         [&](Traversal<K>& tr, uint32_t i, bool fin, bool found) {
@@ -1013,6 +1016,7 @@ __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int he
             uPix.x = lcg(rng); uPix.y = lcg(rng);
             uLens.x = lcg(rng); uLens.y = lcg(rng);
             tr.start(camera_ray(S, x, y, uPix, uLens), kInf, S.exact_only != 0);
+            tr.take_tlas_root();
         },
         [&](Traversal<kStackK>& tr, uint32_t p, bool fin, bool) {
             if (fin) {

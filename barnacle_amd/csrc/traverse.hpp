@@ -97,7 +97,32 @@ struct Traversal {
     }
 
     // No traversal at all (placeholder rays, NaN t max).
-    __device__ __forceinline__ void empty() { st.sp = 0; nxt = kNoNode; }
+    __device__ __forceinline__ void empty() {
+        st.sp = 0;
+        nxt = kNoNode;
+        tl_cur = tl_end = 0;
+    }
+    // The fast traversal tests a root when it is reached: the TLAS root when
+    // the ray starts, a BLAS root when its instance is entered.  A failing
+    // root leaves nothing to do, a leaf root hands over its items at once,
+    // and an interior root is expanded next without a second test.  (The
+    // counting traversal leaves the TLAS root to step(), which tests and
+    // counts it.)
+    __device__ __forceinline__ void take_tlas_root() { take_root(G.tlas, false); }
+    __device__ __forceinline__ void take_root(const float4* base, bool in_blas) {
+        const float4 a = base[0], b = base[1];
+        nxt = kNoNode;
+        if (!box(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y))) return;
+        const uint32_t f = __float_as_uint(b.w);
+        if (f & 0xff) {
+            const int cnt = (int)(int8_t)((f >> 16) & 0xff);
+            const int w = __float_as_int(b.z);
+            if (in_blas) { bl_cur = tri_base + w; bl_end = bl_cur + cnt; }
+            else { tl_cur = w; tl_end = w + cnt; }
+        } else {
+            nxt = 0u;
+        }
+    }
     // An idle lane: no item left anywhere, so next_kind() reports kItemNone
     // and a step does nothing.
     __device__ __forceinline__ void clear() {
@@ -273,11 +298,7 @@ struct Traversal {
             co = mk(wo.x + 0.0f, wo.y + 0.0f, wo.z + 0.0f);
             cd = wd;
             cinv = winv;
-            cur_inst = idx;
-            blas_root = info.x;
-            tri_base = info.y;
-            blas_base = st.sp;
-            nxt = kNeedTest;
+            enter_blas(idx, info);
             return true;
         }
         const Xf w2o = I.w2o;
@@ -295,13 +316,21 @@ struct Traversal {
             co = oo; cd = od;
             cinv = {1.0f / od.x, 1.0f / od.y, 1.0f / od.z};
             exact = exact || !finite3(co) || !finite3(cinv);
-            cur_inst = idx;
-            blas_root = info.x;
-            tri_base = info.y;
-            blas_base = st.sp;
-            nxt = kNeedTest;
+            enter_blas(idx, info);
         }
         return true;
+    }
+
+    // Entry into the BLAS of instance idx with the object ray set; its root
+    // is tested at once, as the reference pops and tests it first
+    // (Mesh.fs:217-242).  A lane whose root fails leaves the BLAS in its next
+    // step.
+    __device__ __forceinline__ void enter_blas(int idx, int4 info) {
+        cur_inst = idx;
+        blas_root = info.x;
+        tri_base = info.y;
+        blas_base = st.sp;
+        take_root(G.blas + 2 * (size_t)info.x, true);
     }
 
     // One item of the fast traversal (all lanes of a wave advance together,
@@ -326,12 +355,8 @@ struct Traversal {
         if (nxt != kNoNode) { e = nxt; nxt = kNoNode; }
         else e = st.pop();
         const float4* base = blas_base >= 0 ? G.blas + 2 * (size_t)blas_root : G.tlas;
-        const int node = (int)(e & kNodeMask);
+        const int node = (int)e;
         const float4 b = base[2 * node + 1];
-        if (e & kNeedTest) {
-            const float4 a = base[2 * node];
-            if (!box(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y))) return;
-        }
         const int word = __float_as_int(b.z);
         const uint32_t f = __float_as_uint(b.w);
         if (f & 0xff) {
@@ -348,11 +373,21 @@ struct Traversal {
         const bool left_near = comp(cd, axis) > 0.0f;
         const int near = left_near ? l : r, far = left_near ? r : l;
         const bool hn = left_near ? hl : hr, hf = left_near ? hr : hl;
-        if (hn) {
-            nxt = (uint32_t)near;
-            if (hf) st.push((uint32_t)far);
-        } else if (hf) {
-            nxt = (uint32_t)far;
+        if (hn && hf) st.push((uint32_t)far);
+        if (hn || hf) {
+            // The child taken next: a leaf hands over its items at once, which
+            // saves the step that would only pop it (same items, same order).
+            const bool take_left = hn ? left_near : !left_near;
+            const float4 tb = take_left ? lb : rb;
+            const uint32_t tf = __float_as_uint(tb.w);
+            if (tf & 0xff) {
+                const int cnt = (int)(int8_t)((tf >> 16) & 0xff);
+                const int w = __float_as_int(tb.z);
+                if (blas_base < 0) { tl_cur = w; tl_end = w + cnt; }
+                else { bl_cur = tri_base + w; bl_end = bl_cur + cnt; }
+            } else {
+                nxt = (uint32_t)(take_left ? l : r);
+            }
         }
     }
 };
