@@ -304,6 +304,8 @@ SceneRec& get_scene(Ctx* c, int id) {
 }
 
 int grid_trace(Ctx* c) { return c->n_cu * c->trace_blocks_per_cu; }
+// Batch rounds: no more blocks per CU than their trace kernels keep resident.
+int grid_trace_batch(Ctx* c) { return c->n_cu * std::min(c->trace_blocks_per_cu, bnd::kTraceWavesBatch); }
 
 void ensure_paths(Ctx* c, size_t cap) {
     if (cap <= c->path_cap) return;
@@ -674,7 +676,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     L.hdr = (const bnd::RoundHeader*)c->b_hdr.p;
     L.items = (const bnd::RoundItem*)c->b_items.p;
     L.scenes = (const bnd::DevScene*)c->b_scenes.p;
-    L.grid_trace = grid_trace(c);
+    L.grid_trace = grid_trace_batch(c);
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.stream = s;
@@ -824,7 +826,7 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
         if ((size_t)v >= slots_total) break;          // larger chunks change nothing
     }
     if (chunks.empty()) chunks.push_back(chunk);
-    std::vector<Knob> knobs = {{&blocks, {3, 4, 5, 6, 8}}, {&refill, {24, 32, 40, 48}}, {&stack_k, {16, 32}},
+    std::vector<Knob> knobs = {{&blocks, {4, 5, 6, 7, 8}}, {&refill, {24, 32, 40, 48}}, {&stack_k, {16, 32}},
                                {&chunk, chunks}};
     apply();
     double best = measure();

@@ -76,7 +76,7 @@ struct Ctx {
     int device = 0;
     int n_cu = 1;
     hipStream_t stream = nullptr;
-    int trace_blocks_per_cu = 6;
+    int trace_blocks_per_cu = 7;
     int refill = 32;                      // idle lanes that trigger a refill
     size_t chunk_slots = (size_t)64 << 20;       // path slots in flight
     size_t window_paths = 0;                     // paths per round; 0: as many as slots

@@ -17,6 +17,12 @@ constexpr int kSegStride = 64;    // words between two fetch counters (own cache
 
 constexpr uint32_t kNoPath = 0xffffffffu;
 
+// Waves per SIMD the 16-entry-window trace kernels are built for: seven for
+// one scene (72 VGPRs), six in batches, whose per-lane scene pointers need
+// the 80 VGPRs of six waves to avoid spilling.
+constexpr int kTraceWaves = 7;
+constexpr int kTraceWavesBatch = 6;
+
 // Path state, struct of arrays indexed by slot.
 struct PathBuf {
     float4* o;        // ray origin, w = bounce depth of the path (int bits)

@@ -344,10 +344,11 @@ __device__ __forceinline__ void persistent_trace(SegFetch F, uint32_t count, uin
 // material bin of the hit (kind of the instance's material; light-only
 // instances in the last bin, -1 for a miss) travels in the hit record.  The
 // rays are queue[i], or position i of a compact round (queue null).
-// With a 16-entry LDS stack window six blocks fit a CU's LDS; the register
-// budget is then set for six waves per SIMD.
+// With a 16-entry LDS stack window seven blocks fit a CU's LDS; the register
+// budget is then set for seven waves per SIMD (72 VGPRs; eight would spill),
+// six in batches.
 template <bool COUNT, bool BATCH, int K>
-__global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
+__global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWaves) : 1) void k_trace_closest(Work<BATCH> W, PathBuf P, Ctl* ctl, int cur,
                                                        const uint32_t* queue, uint32_t* spill, size_t spill_stride,
                                                        uint32_t refill, unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -661,7 +662,7 @@ __global__ __launch_bounds__(256) void k_shade_c(Work<BATCH> W, PathBuf X, PathS
 // rays of the slots in shq, added to the slot's L.  Compact rounds (shq
 // null): ray i, added to L_next[target] or to L_out[path id] (nee_a.w).
 template <bool COUNT, bool BATCH, int K>
-__global__ __launch_bounds__(256, K == 16 ? 6 : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
+__global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWaves) : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                       float4* L_next, float4* L_out,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {

@@ -34,7 +34,7 @@ def test_launch_parameters_do_not_change_film(ctx, mesh_job, opts):
         got = ds.render_pt(p)
         films, _ = ctx.render_batch([(ds, p), (ds, p)])
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 32, "chunk_slots": 64 << 20,
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 32, "chunk_slots": 64 << 20,
                      "material_sort": -1}.items():
             ctx.set_option(k, v)
     assert np.array_equal(got.view(np.int32), ref.view(np.int32))
@@ -47,7 +47,7 @@ def test_autotune(ctx, mesh_job):
     ref = ds.render_pt(p)
     try:
         r = ctx.autotune([(ds, p)])
-        assert r["trace_blocks_per_cu"] in (3, 4, 5, 6, 8)
+        assert r["trace_blocks_per_cu"] in (4, 5, 6, 7, 8)
         assert r["refill"] in (24, 32, 40, 48)
         assert r["stack_k"] in (16, 32)
         assert r["trials"] >= 8
@@ -56,6 +56,6 @@ def test_autotune(ctx, mesh_job):
         rb = ctx.autotune([(ds, p), (ds, p)], batched=True)
         assert rb["trials"] >= 8
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 6, "refill": 32, "chunk_slots": 64 << 20,
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 32, "chunk_slots": 64 << 20,
                      "material_sort": -1}.items():
             ctx.set_option(k, v)
