@@ -596,6 +596,13 @@ void backward_pt(Ctx* c, SceneRec& sc, const bn_pt_params& p, const float* d_fil
     B.adj.d_film = (const float*)c->a_dfilm.p;
     B.adj.g_mat = (float*)c->a_gmat.p;
     B.adj.g_light = (float*)c->a_glight.p;
+    B.adj.n_mat = n_mat;
+    B.adj.n_light = n_light;
+    {
+        // per-block gradient sums in LDS when the tables fit a workgroup's 160 KB
+        const size_t bytes = (4 * (size_t)n_mat + 3 * (size_t)n_light) * sizeof(float);
+        B.adj.lds_bytes = bytes <= 160 * 1024 - 1024 ? (int)bytes : 0;
+    }
     // The forward pass of the same frame may still be in L_out (the last
     // bn_render_pt, one round, same scene parameters): then only replay.
     const bool cached = c->fwd.valid && c->fwd.scene == &sc && c->fwd.gen == sc.gen && ns >= p.spp &&

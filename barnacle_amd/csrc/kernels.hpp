@@ -113,6 +113,8 @@ struct AdjBuf {
     const float* d_film;      // W*H*3 film adjoint, Film.Pixels order
     float* g_mat;             // 4 per material: albedo rgb, roughness
     float* g_light;           // 4 per light definition: emission rgb, unused
+    int n_mat, n_light;       // table sizes
+    int lds_bytes;            // > 0: NEE gradients summed per block in this much LDS first
 };
 
 struct Buffers {

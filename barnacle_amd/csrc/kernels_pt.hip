@@ -445,10 +445,10 @@ __global__ __launch_bounds__(256) void k_bin(PathBuf P, Ctl* ctl, int cur, const
 
 // Block-aggregated positions in N queues: one global atomic per queue and
 // block.  pos[o] is the thread's position in queue o when pred[o] holds.
-// Every thread of the (256-thread) block must call it.
-template <int N>
+// Every thread of the block (NW waves) must call it.
+template <int N, int NW = 4>
 __device__ __forceinline__ void block_pos(const bool (&pred)[N], uint32_t* const (&ctr)[N], uint32_t (&pos)[N]) {
-    __shared__ uint32_t wcnt[N][5];
+    __shared__ uint32_t wcnt[N][NW + 1];
     const int w = (int)(threadIdx.x >> 6);
     uint64_t m[N];
     for (int o = 0; o < N; ++o) {
@@ -459,24 +459,24 @@ __device__ __forceinline__ void block_pos(const bool (&pred)[N], uint32_t* const
     if (threadIdx.x < (uint32_t)N) {
         const int o = (int)threadIdx.x;
         uint32_t run = 0;
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < NW; ++v) {
             const uint32_t c = wcnt[o][v];
             wcnt[o][v] = run;
             run += c;
         }
-        wcnt[o][4] = run ? atomicAdd(ctr[o], run) : 0u;
+        wcnt[o][NW] = run ? atomicAdd(ctr[o], run) : 0u;
     }
     __syncthreads();
-    for (int o = 0; o < N; ++o) pos[o] = wcnt[o][4] + wcnt[o][w] + mbcnt(m[o]);
+    for (int o = 0; o < N; ++o) pos[o] = wcnt[o][NW] + wcnt[o][w] + mbcnt(m[o]);
     __syncthreads();
 }
 
 // Block-aggregated append of `val` to N queues.
-template <int N>
+template <int N, int NW = 4>
 __device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* const (&ctr)[N],
                                            uint32_t* const (&arr)[N], uint32_t val) {
     uint32_t pos[N];
-    block_pos<N>(pred, ctr, pos);
+    block_pos<N, NW>(pred, ctr, pos);
     for (int o = 0; o < N; ++o)
         if (pred[o]) arr[o][pos[o]] = val;
 }
@@ -720,6 +720,32 @@ __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWa
 //
 // Gradients are summed per wave for lanes sharing a key (material or light)
 // before one atomic per key, and lanes with rare keys add directly.
+// Per-block gradient tables in LDS (4 floats per material, then 3 per light),
+// flushed to the global tables (4 floats per entry) once per block.
+constexpr int kGradLdsThreads = 1024;
+__device__ __forceinline__ void lds_grad_clear(float* s, int n_mat, int n_light) {
+    for (int k = threadIdx.x; k < 4 * n_mat + 3 * n_light; k += blockDim.x) s[k] = 0.0f;
+    __syncthreads();
+}
+__device__ __forceinline__ void lds_grad_mat(float* s, int key, float4 v, int n_mat) {
+    if (key < 0 || key >= n_mat) return;
+    float* d = s + 4 * key;
+    atomicAdd(d + 0, v.x); atomicAdd(d + 1, v.y); atomicAdd(d + 2, v.z); atomicAdd(d + 3, v.w);
+}
+__device__ __forceinline__ void lds_grad_light(float* s, int key, float4 v, int n_mat, int n_light) {
+    if (key < 0 || key >= n_light) return;
+    float* d = s + 4 * n_mat + 3 * key;
+    atomicAdd(d + 0, v.x); atomicAdd(d + 1, v.y); atomicAdd(d + 2, v.z);
+}
+__device__ __forceinline__ void lds_grad_flush(const float* s, float* g_mat, float* g_light, int n_mat, int n_light) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < 4 * n_mat; k += blockDim.x)
+        if (s[k] != 0.0f) unsafeAtomicAdd(&g_mat[k], s[k]);
+    const float* sl = s + 4 * n_mat;
+    for (int k = threadIdx.x; k < 3 * n_light; k += blockDim.x)
+        if (sl[k] != 0.0f) unsafeAtomicAdd(&g_light[4 * (k / 3) + k % 3], sl[k]);
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -755,9 +781,13 @@ __device__ __forceinline__ float ratio(float d, float f) { return f != 0.0f ? d 
 // with the terms its gradient needs (applied by k_trace_shadow_adj when the
 // shadow ray is unoccluded); the BSDF sample and its term, applied here when
 // no shadow ray follows and after the NEE subtraction otherwise.
-__global__ __launch_bounds__(256) void k_shade_adj(Work<false> W, PathBuf P, AdjBuf A, Ctl* ctl,
-                                                   const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next,
-                                                   int nxt, uint32_t* shq, uint32_t* freel) {
+// LDS: 1024-thread blocks sum the gradients in LDS (see lds_grad_*).
+template <bool LDS>
+__global__ __launch_bounds__(LDS ? kGradLdsThreads : 256) void k_shade_adj(
+        Work<false> W, PathBuf P, AdjBuf A, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next,
+        int nxt, uint32_t* shq, uint32_t* freel) {
+    extern __shared__ float lds_g[];
+    if (LDS) lds_grad_clear(lds_g, A.n_mat, A.n_light);
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
@@ -863,19 +893,27 @@ __global__ __launch_bounds__(256) void k_shade_adj(Work<false> W, PathBuf P, Adj
             P.L[slot] = f4(L, 0.0f);
             fin = !cont;
         }
-        grad_accum(A.g_light, light_key, g_l, light_key >= 0);
-        grad_accum(A.g_mat, prev_key, g_p, prev_key >= 0);
-        grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
-        block_push<3>({cont, shadow, fin}, {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
-                      {queue_next, shq, freel}, slot);
+        if (LDS) {
+            lds_grad_light(lds_g, light_key, g_l, A.n_mat, A.n_light);
+            lds_grad_mat(lds_g, prev_key, g_p, A.n_mat);
+            lds_grad_mat(lds_g, mat_key, g_m, A.n_mat);
+        } else {
+            grad_accum(A.g_light, light_key, g_l, light_key >= 0);
+            grad_accum(A.g_mat, prev_key, g_p, prev_key >= 0);
+            grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
+        }
+        block_push<3, LDS ? kGradLdsThreads / 64 : 4>({cont, shadow, fin},
+                                                      {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
+                                                      {queue_next, shq, freel}, slot);
     }
+    if (LDS) lds_grad_flush(lds_g, A.g_mat, A.g_light, A.n_mat, A.n_light);
 }
 
 // The shadow rays of a replayed bounce only record their visibility (in
 // nee_a.w): gradient sums belong in a kernel whose lanes are all busy, not in
 // the traversal loop.
 template <int K>
-__global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
+__global__ __launch_bounds__(256, K == 16 ? kTraceWaves : 1) void k_trace_shadow_vis(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
                                                           uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                           uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -900,44 +938,64 @@ __global__ __launch_bounds__(256) void k_trace_shadow_vis(Work<false> W, PathBuf
 // radiance still to come and adds its gradients (the vertex's material
 // through f and the NEE MIS weight, the sampled light through its emission);
 // then the vertex's BSDF-sample term is applied.
+__device__ __forceinline__ void nee_adj_item(const PathBuf& P, const AdjBuf& A, uint32_t slot, int& mat_key,
+                                             int& light_key, float4& g_m, float4& g_l) {
+    V3 L = xyz(P.L[slot]);
+    const V3 dlt = xyz(A.delta[slot]);
+    const float4 a = P.nee_a[slot];
+    V3 gm = mk(0, 0, 0);
+    float gr = 0.0f;
+    if (a.w != 0.0f) {
+        const float4 b = P.nee_b[slot];
+        const float4 da = A.nee_da[slot], dr = A.nee_dr[slot];
+        const V3 c = xyz(a) * xyz(b);
+        L = L - c;
+        gm = dlt * xyz(da) * xyz(b);
+        // d (f Le / (pdf_b + pdf_l)) / d rough = f' Le / S - c pdf_b' / S
+        gr = dot(dlt, xyz(dr) * xyz(b) - c * P.sh_d[slot].w);
+        mat_key = __float_as_int(da.w);
+        light_key = __float_as_int(dr.w);
+        g_l = f4(dlt * xyz(a) * b.w, 0.0f);
+    }
+    const float4 pa = A.pend_a[slot];
+    const int pend = __float_as_int(pa.w);
+    if (pend >= 0) {
+        gm = gm + dlt * xyz(pa) * L;
+        gr += dot(dlt * xyz(A.pend_r[slot]), L);
+        mat_key = pend;
+    }
+    g_m = f4(gm, gr);
+    P.L[slot] = f4(L, 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_nee_adj(PathBuf P, AdjBuf A, const Ctl* ctl, const uint32_t* shq) {
     const uint32_t count = ctl->sh_count;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         int mat_key = -1, light_key = -1;
         float4 g_m = make_float4(0, 0, 0, 0), g_l = make_float4(0, 0, 0, 0);
-        if (i < count) {
-            const uint32_t slot = shq[i];
-            V3 L = xyz(P.L[slot]);
-            const V3 dlt = xyz(A.delta[slot]);
-            const float4 a = P.nee_a[slot];
-            V3 gm = mk(0, 0, 0);
-            float gr = 0.0f;
-            if (a.w != 0.0f) {
-                const float4 b = P.nee_b[slot];
-                const float4 da = A.nee_da[slot], dr = A.nee_dr[slot];
-                const V3 c = xyz(a) * xyz(b);
-                L = L - c;
-                gm = dlt * xyz(da) * xyz(b);
-                // d (f Le / (pdf_b + pdf_l)) / d rough = f' Le / S - c pdf_b' / S
-                gr = dot(dlt, xyz(dr) * xyz(b) - c * P.sh_d[slot].w);
-                mat_key = __float_as_int(da.w);
-                light_key = __float_as_int(dr.w);
-                g_l = f4(dlt * xyz(a) * b.w, 0.0f);
-            }
-            const float4 pa = A.pend_a[slot];
-            const int pend = __float_as_int(pa.w);
-            if (pend >= 0) {
-                gm = gm + dlt * xyz(pa) * L;
-                gr += dot(dlt * xyz(A.pend_r[slot]), L);
-                mat_key = pend;
-            }
-            g_m = f4(gm, gr);
-            P.L[slot] = f4(L, 0.0f);
-        }
+        if (i < count) nee_adj_item(P, A, shq[i], mat_key, light_key, g_m, g_l);
         grad_accum(A.g_mat, mat_key, g_m, mat_key >= 0);
         grad_accum(A.g_light, light_key, g_l, light_key >= 0);
     }
+}
+
+// The same with the gradients of the whole grid's share summed in LDS first:
+// device-scope float atomics on a small table serialise, LDS atomics do not.
+// One 1024-thread block per CU; used when the tables fit a workgroup's LDS.
+__global__ __launch_bounds__(kGradLdsThreads) void k_nee_adj_lds(PathBuf P, AdjBuf A, const Ctl* ctl,
+                                                                const uint32_t* shq) {
+    extern __shared__ float lds_g[];
+    lds_grad_clear(lds_g, A.n_mat, A.n_light);
+    const uint32_t count = ctl->sh_count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        int mat_key = -1, light_key = -1;
+        float4 g_m = make_float4(0, 0, 0, 0), g_l = make_float4(0, 0, 0, 0);
+        nee_adj_item(P, A, shq[i], mat_key, light_key, g_m, g_l);
+        lds_grad_mat(lds_g, mat_key, g_m, A.n_mat);
+        lds_grad_light(lds_g, light_key, g_l, A.n_mat, A.n_light);
+    }
+    lds_grad_flush(lds_g, A.g_mat, A.g_light, A.n_mat, A.n_light);
 }
 
 // -------------------------------------------------------------------- film
@@ -1151,8 +1209,16 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
         } else if (kind == Integrator::Adjoint) {
             if constexpr (!BATCH) {
                 { Bracket b(hook, "shade_adj", st);
-                  hipLaunchKernelGGL(k_shade_adj, dim3(L.grid_shade), dim3(256), 0, st, W, P, B.adj, ctl, B.bins,
-                                     B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
+                  if (B.adj.lds_bytes > 0) {
+                      hipFuncSetAttribute((const void*)k_shade_adj<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          B.adj.lds_bytes);
+                      hipLaunchKernelGGL(k_shade_adj<true>, dim3(L.grid_shade / 8), dim3(kGradLdsThreads),
+                                         B.adj.lds_bytes, st, W, P, B.adj, ctl, B.bins, B.bin_cap, B.queue[nxt], nxt,
+                                         B.shq, B.freel[nxt]);
+                  } else {
+                      hipLaunchKernelGGL(k_shade_adj<false>, dim3(L.grid_shade), dim3(256), 0, st, W, P, B.adj, ctl,
+                                         B.bins, B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]);
+                  } }
                 { Bracket b(hook, "trace_shadow", st);
                   uint32_t* seg = B.fetch_seg + kFetchSegs * kSegStride;
                   if (B.stack_k == 16)
@@ -1162,7 +1228,14 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
                       hipLaunchKernelGGL(k_trace_shadow_vis<32>, dim3(L.grid_trace), dim3(256), 0, st, W, P, ctl,
                                          B.shq, B.spill, B.spill_stride, L.refill, seg); }
                 { Bracket b(hook, "nee_adj", st);
-                  hipLaunchKernelGGL(k_nee_adj, dim3(L.grid_shade), dim3(256), 0, st, P, B.adj, ctl, B.shq); }
+                  if (B.adj.lds_bytes > 0) {
+                      hipFuncSetAttribute((const void*)k_nee_adj_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          B.adj.lds_bytes);
+                      hipLaunchKernelGGL(k_nee_adj_lds, dim3(L.grid_shade / 8), dim3(kGradLdsThreads),
+                                         B.adj.lds_bytes, st, P, B.adj, ctl, B.shq);
+                  } else {
+                      hipLaunchKernelGGL(k_nee_adj, dim3(L.grid_shade), dim3(256), 0, st, P, B.adj, ctl, B.shq);
+                  } }
             }
         } else {
             { Bracket b(hook, "shade", st);
