@@ -604,51 +604,76 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 // in L_out[path id].  Shadow rays are stored by their own position, with the
 // item (batch) in sh_d.w and the target of the NEE term in nee_a.w.  The
 // vertices shaded are counted in bin_count[0].
+// The vertex results wait in LDS while the block computes the output
+// positions, so they do not occupy registers across that step (the kernel
+// would otherwise need about 160 VGPRs, three waves per SIMD).
+struct ShadeStage {
+    float4 o[256], d[256], beta[256], L[256];     // L.w: rng bits
+    float4 sh_o[256], sh_d[256], nee_a[256], nee_b[256];
+};
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur, int nxt,
-                                                 float4* L_out) {
+__global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
+                                                    int nxt, float4* L_out) {
+    __shared__ ShadeStage sg;
     const uint32_t total = ctl->q_count[cur];
+    const int t = (int)threadIdx.x;
     uint32_t shaded = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
-        Vertex v;
-        v.cont = false;
-        v.shadow = false;
+        bool cont = false, shadow = false;
         uint32_t pid = 0;
         int k = 0;
         if (i < total) {
             pid = X.pid[i];
             k = W.item_of(X, i);
             const int4 h = X.hit[i];
+            Vertex v;
+            v.cont = false;
+            v.shadow = false;
             if (h.x >= 0) {
                 ++shaded;
                 shade_vertex(W.scene(k), W.item(k), X, i, h, v);
             } else {
                 v.L = xyz(X.L[i]);
             }
+            cont = v.cont;
+            shadow = v.shadow;
+            sg.L[t] = f4(v.L, __uint_as_float(v.rng));
+            if (cont) {
+                sg.o[t] = v.o;
+                sg.d[t] = v.d;
+                sg.beta[t] = v.beta;
+            }
+            if (shadow) {
+                sg.sh_o[t] = v.sh_o;
+                sg.sh_d[t] = v.sh_d;
+                sg.nee_a[t] = v.nee_a;
+                sg.nee_b[t] = v.nee_b;
+            }
         }
         uint32_t pos[2];
-        block_pos<2>({v.cont, v.shadow}, {&ctl->q_count[nxt], &ctl->sh_count}, pos);
+        block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->sh_count}, pos);
         if (i < total) {
-            if (v.cont) {
+            const float4 l = sg.L[t];
+            if (cont) {
                 const uint32_t j = pos[0];
-                Y.o[j] = v.o;
-                Y.d[j] = v.d;
-                Y.beta[j] = v.beta;
-                Y.L[j] = f4(v.L, 0.0f);
-                Y.rng[j] = v.rng;
+                Y.o[j] = sg.o[t];
+                Y.d[j] = sg.d[t];
+                Y.beta[j] = sg.beta[t];
+                Y.L[j] = make_float4(l.x, l.y, l.z, 0.0f);
+                Y.rng[j] = __float_as_uint(l.w);
                 Y.pid[j] = pid;
                 if (BATCH) Y.item[j] = k;
             } else {
-                L_out[pid] = f4(v.L, 0.0f);
+                L_out[pid] = make_float4(l.x, l.y, l.z, 0.0f);
             }
-            if (v.shadow) {
+            if (shadow) {
                 const uint32_t q = pos[1];
-                X.sh_o[q] = v.sh_o;
-                X.sh_d[q] = make_float4(v.sh_d.x, v.sh_d.y, v.sh_d.z, __int_as_float(k));
-                X.nee_a[q] = make_float4(v.nee_a.x, v.nee_a.y, v.nee_a.z,
-                                         __uint_as_float(v.cont ? pos[0] : (kTargetDone | pid)));
-                X.nee_b[q] = v.nee_b;
+                const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
+                X.sh_o[q] = sg.sh_o[t];
+                X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
+                X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(cont ? pos[0] : (kTargetDone | pid)));
+                X.nee_b[q] = sg.nee_b[t];
             }
         }
     }
