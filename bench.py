@@ -99,7 +99,28 @@ def batched(ctx, n=32):
                       "256^2-1024^2, 4-64 spp, depth 4-16",
             "paths": paths, "plain_s": plain_s, "batched_s": batch_s, "speedup": plain_s / batch_s,
             "batched_mpaths_per_s": paths / batch_s / 1e6, "rounds": st["rounds"], "graph": st["graph"],
-            "films_bit_equal": equal, "scene_setup_s": setup}
+            "films_bit_equal": equal, "scene_setup_s": setup, "small_items": small_items(ctx)}
+
+
+def small_items(ctx, n=256):
+    """Many small render items (the serving case batching is for): n frames
+    of the Cornell box at 128x128, 4 spp, depth 8, frame ids 0..n-1, one at a
+    time against one batched render."""
+    sc = scene.load_scene(configs.cornell_box(128, 128, 4, 8))
+    d = ctx.scene(sc)
+    jobs = [(d, sc.pt_params(frame_id=f)) for f in range(n)]
+    for j in jobs[:4]:
+        j[0].render_pt(j[1])
+    t0 = time.perf_counter()
+    plain = [dd.render_pt(p) for dd, p in jobs]
+    plain_s = time.perf_counter() - t0
+    ctx.render_batch(jobs)
+    t0 = time.perf_counter()
+    films, st = ctx.render_batch(jobs)
+    batch_s = time.perf_counter() - t0
+    equal = all(np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(films, plain))
+    return {"config": f"{n} frames of the Cornell box, 128x128, 4 spp, depth 8", "plain_s": plain_s,
+            "batched_s": batch_s, "speedup": plain_s / batch_s, "films_bit_equal": equal}
 
 
 def adjoint(ctx, steps=12):
