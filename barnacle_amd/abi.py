@@ -100,7 +100,7 @@ class BatchItem(C.Structure):
 
 
 class BatchStats(C.Structure):
-    _fields_ = [("rounds", i32), ("graph", i32)]
+    _fields_ = [("rounds", i32), ("graph", i32), ("scene_rounds", i32)]
 
 
 class TuneResult(C.Structure):

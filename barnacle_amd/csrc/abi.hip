@@ -409,6 +409,7 @@ void check_pt_params(const bn_pt_params& p) {
 }
 
 constexpr int kIterBlock = 8;     // iterations between two drain checks (even)
+constexpr size_t kMaxGraphs = 64;  // captured graphs kept per context
 
 // Paths per round.  By default a round holds no more paths than there are
 // slots, so every path of a round is in flight from the first iteration:
@@ -684,30 +685,48 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     L.items = (const bnd::RoundItem*)c->b_items.p;
     L.scenes = (const bnd::DevScene*)c->b_scenes.p;
     L.grid_trace = grid_trace_batch(c);
+    L.grid_uniform = grid_trace(c);
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.stream = s;
     L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
-    hipGraphExec_t exec = nullptr;
-    if (c->use_graph) {
-        const std::string key = "block:" + std::to_string(L.grid_trace) + ":" + std::to_string(L.refill) + ":" +
-                                std::to_string(B.stack_k) + ":" + std::to_string(L.n_slots) + ":" +
-                                std::to_string((uintptr_t)c->b_items.p) + ":" +
-                                std::to_string((uintptr_t)c->b_scenes.p) + ":" + std::to_string((uintptr_t)c->L_out.p);
+    // One captured block of iterations per trace binding: per ray (mixed
+    // rounds), or one scene's geometry in the launch arguments (rounds over
+    // one scene; keyed by everything of the scene that the trace kernels read).
+    auto block_graph = [&](const bnd::Launch& Lr) -> hipGraphExec_t {
+        if (!c->use_graph) return nullptr;
+        std::string key = "block:" + std::to_string(L.grid_trace) + ":" + std::to_string(L.refill) + ":" +
+                          std::to_string(B.stack_k) + ":" + std::to_string(L.n_slots) + ":" +
+                          std::to_string((uintptr_t)c->b_items.p) + ":" + std::to_string((uintptr_t)c->b_scenes.p) +
+                          ":" + std::to_string((uintptr_t)c->L_out.p);
+        if (Lr.uniform) {
+            const bnd::DevScene& d = Lr.scene;
+            for (const void* q : {(const void*)d.tlas, (const void*)d.blas, (const void*)d.tris, (const void*)d.insts,
+                                  (const void*)d.mats})
+                key += ":" + std::to_string((uintptr_t)q);
+            key += ":" + std::to_string(d.exact_only) + ":" + std::to_string(Lr.grid_uniform);
+        }
         auto f = c->graphs.find(key);
         if (f == c->graphs.end()) {
+            if (c->graphs.size() >= kMaxGraphs) c->graphs.clear();   // no earlier round's graph is in flight
             hipGraph_t g;
             hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-            bnd::enqueue_iterations(true, bnd::Integrator::Path, L, P, ctl, B, kIterBlock, false, nullptr);
+            bnd::enqueue_iterations(true, bnd::Integrator::Path, Lr, P, ctl, B, kIterBlock, false, nullptr);
             hip_check(hipStreamEndCapture(s, &g), "end capture");
             hipGraphExec_t e;
             hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "graph instantiate");
             hip_check(hipGraphDestroy(g), "graph destroy");
             f = c->graphs.emplace(key, GraphExec(e)).first;
         }
-        exec = f->second.e;
-    }
+        return f->second.e;
+    };
+    // An item with at least an eighth of a round of paths gets rounds over
+    // its own scene; smaller items share rounds whatever their scenes, so
+    // their bounce tails fill each other.
+    auto big = [&](int k) { return (size_t)base[k].p_pad * (size_t)items[k].params.spp * 8 >= cap; };
+    bool used_graph = false;
+    int scene_rounds = 0;
     Hook hook(c, ctl, nullptr);
     std::vector<int> next(n, 0);
     std::vector<bnd::RoundItem> round;
@@ -718,13 +737,19 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         int active = 0;
         for (int k = 0; k < n; ++k) active += next[k] < items[k].params.spp;
         if (active == 0) break;
-        // Item-major packing keeps each round on few scenes (their BVHs stay
-        // in cache); fair-share packing gives every unfinished item a slice.
+        // Item-major packing, from the first unfinished item: the samples of
+        // its scene's items when it is big, else of the small items; rounds
+        // hold whole samples.  Fair-share packing gives every unfinished item
+        // a slice.
         std::vector<int> take(n, 0);
         if (c->batch_pack == 0) {
+            int first = 0;
+            while (next[first] >= items[first].params.spp) ++first;
+            const bool own = big(first);
             for (int k = 0; k < n; ++k) {
                 const int left = items[k].params.spp - next[k];
                 if (left <= 0) continue;
+                if (own ? items[k].scene != items[first].scene : big(k)) continue;
                 const int t = (int)std::min<size_t>((size_t)left, (cap - used) / (size_t)base[k].p_pad);
                 if (t <= 0) break;
                 take[k] = t;
@@ -746,8 +771,12 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
                 }
         }
         int path = 0, pix = 0, round_depth = 0;
+        L.uniform = true;
+        int k0 = -1;
         for (int k = 0; k < n; ++k) {
             if (take[k] == 0) continue;
+            if (k0 < 0) k0 = k;
+            L.uniform = L.uniform && items[k].scene == items[k0].scene;
             bnd::RoundItem it = base[k];
             it.s0 = next[k];
             it.ns = take[k];
@@ -766,6 +795,10 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         hip_check(hipMemcpyAsync(c->b_items.p, round.data(), round.size() * sizeof(bnd::RoundItem),
                                  hipMemcpyHostToDevice, s), "round table");
         hip_check(hipMemcpyAsync(c->b_hdr.p, &h, sizeof(h), hipMemcpyHostToDevice, s), "round header");
+        L.scene = scenes[k0];
+        scene_rounds += L.uniform;
+        const hipGraphExec_t exec = block_graph(L);
+        used_graph = used_graph || exec;
         run_round(c, true, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)path, round_depth, false, &hook, exec);
         hip_check(hipStreamSynchronize(s), "round");
         ++rounds;
@@ -780,7 +813,8 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     c->timer.resolve();
     if (stats) {
         stats->rounds = rounds;
-        stats->graph = exec ? 1 : 0;
+        stats->graph = used_graph ? 1 : 0;
+        stats->scene_rounds = scene_rounds;
     }
 }
 

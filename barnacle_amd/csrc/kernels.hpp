@@ -98,6 +98,11 @@ struct Launch {
     uint32_t n_pix;           // plain mode
     uint32_t n_slots;         // path slots in flight
     int grid_trace, grid_shade;
+    // Batch mode: every item of the round traces the geometry of `scene`, so
+    // its compact rounds launch the single-scene trace kernels (the scene
+    // bound once per launch) on grid_uniform blocks.
+    bool uniform;
+    int grid_uniform;
     uint32_t refill;
     hipStream_t stream;
 };

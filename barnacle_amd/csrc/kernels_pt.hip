@@ -1180,6 +1180,9 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
                                  const Buffers& B, int n, bool count, LaunchHook* hook) {
     hipStream_t st = L.stream;
     const PathBuf copies[2] = {P, with_state(P, B.alt)};
+    // The trace kernels read only the geometry of a ray's scene, so a batch
+    // round over one scene traces with the single-scene kernels.
+    const bool one_scene = BATCH && L.uniform;
     for (int i = 0; i < n; ++i) {
         const int cur = i & 1, nxt = 1 - cur;
         const PathBuf& X = copies[cur];
@@ -1190,13 +1193,21 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
           hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg,
                              L.n_slots); }
         { Bracket b(hook, "trace_closest", st);
-          launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
+          if (one_scene)
+              launch_closest_t<false>(count, B.stack_k, plain_work(L), X, ctl, cur, nullptr, B, L.refill,
+                                      L.grid_uniform, st);
+          else
+              launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
         { Bracket b(hook, "shade", st);
           hipLaunchKernelGGL(k_shade_c<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, X, state_of(Y), ctl, cur,
                              nxt, B.L_out); }
         { Bracket b(hook, "trace_shadow", st);
-          launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, Y.L, B.L_out, B, L.refill, L.grid_trace,
-                                 st); }
+          if (one_scene)
+              launch_shadow_t<false>(count, B.stack_k, plain_work(L), X, ctl, nullptr, Y.L, B.L_out, B, L.refill,
+                                     L.grid_uniform, st);
+          else
+              launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, Y.L, B.L_out, B, L.refill, L.grid_trace,
+                                     st); }
         if (hook) hook->after_bounce(cur);
     }
 }

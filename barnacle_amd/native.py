@@ -151,7 +151,7 @@ class Context:
             ptrs[k] = _p(films[k], f32p)
         st = abi.BatchStats()
         check(self.lib.bn_render_batch(self.h, items, n, ptrs, C.byref(st)))
-        return films, {"rounds": st.rounds, "graph": st.graph}
+        return films, {"rounds": st.rounds, "graph": st.graph, "scene_rounds": st.scene_rounds}
 
 
     def autotune(self, jobs, batched: bool = False):

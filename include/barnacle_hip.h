@@ -282,6 +282,8 @@ typedef struct bn_batch_item {
 typedef struct bn_batch_stats {
     int32_t rounds;
     int32_t graph;                     /* 1 when the rounds replayed a graph */
+    int32_t scene_rounds;              /* rounds whose items all share one scene: their
+                                          traversals bind it once per launch */
 } bn_batch_stats;
 int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* const* films,
                     bn_batch_stats* stats);

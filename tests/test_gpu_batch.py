@@ -69,3 +69,26 @@ def test_batch_scene_matches_oracle(ctx, orc, jobs, i):
     rel = np.sqrt(np.sum((g[ok] - o[ok]) ** 2) / max(np.sum(o[ok] ** 2), 1e-30))
     assert rel <= 1e-3
     assert np.mean(g == o) > 0.99
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_scene_rounds_equal_plain(ctx, jobs, graph):
+    """Items with at least an eighth of a round of paths get rounds over their
+    own scene (traced by the single-scene kernels); smaller items of other
+    scenes share a mixed round.  Here: two rounds over the config-1 scene
+    (frames 0 and 1, then frame 2) and one mixed round of three small scenes."""
+    ds_a, _, sc_a = jobs[7]
+    big = [(ds_a, sc_a.pt_params(width=64, height=64, spp=4, frame_id=f)) for f in range(3)]
+    small = [(ds, sc.pt_params(spp=1)) for ds, _, sc in jobs[:3]]
+    assert all(-(-p.width // 8) * -(-p.height // 8) * 64 * p.spp * 8 < 32768 for _, p in small)
+    work = [big[0], small[0], big[1], small[1], small[2], big[2]]
+    ctx.set_option("window_paths", 32768)
+    ctx.set_option("graph", graph)
+    try:
+        films, st = ctx.render_batch(work)
+    finally:
+        ctx.set_option("window_paths", 0)
+        ctx.set_option("graph", 1)
+    assert (st["rounds"], st["scene_rounds"], st["graph"]) == (3, 2, graph)
+    for (ds, p), f in zip(work, films):
+        assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
