@@ -134,6 +134,7 @@ def adjoint(ctx, steps=12):
     (tm, tl), _, _ = configs.config3_params(sc, info)
     ds.set_params(tm, tl)
     target = torch.from_numpy(ds.render_pt(p))
+    _, st_fwd = ds.render_pt(p, stats=True)                 # counts of the timed forward render
     fwd = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -145,10 +146,20 @@ def adjoint(ctx, steps=12):
     iters = np.diff(np.asarray(stamps))[1:]                 # steady state: the first step allocates
     it_s = float(np.median(iters))
     f_s = float(np.median(fwd))
+    # An iteration renders forward and then replays the same paths for the
+    # gradients (the same extension and shadow rays), so its algorithmic bytes
+    # are twice those of a forward render, counted at the last step's parameters.
+    _, st_it = ds.render_pt(p, stats=True)
+    paths = p.width * p.height * p.spp
+    peak = HBM_PEAK_GBS * 1e9
+    f_bytes, it_bytes = frame_bytes(st_fwd), 2 * frame_bytes(st_it)
     return {"config": "config3: 4096-patch GGX sphere in the Cornell box, 512x512, 32 spp, depth 8, "
                       "4096 materials x (albedo rgb, roughness) + 4096 lights x emission rgb",
             "iters_per_s": 1.0 / it_s, "iter_ms": it_s * 1e3, "forward_ms": f_s * 1e3,
             "iter_over_forward": it_s / f_s, "steps": steps, "loss_first": hist[0], "loss_last": hist[-1],
+            "forward_mpaths_per_s": paths / f_s / 1e6, "forward_algorithmic_bytes": f_bytes,
+            "forward_hbm_roofline_frac": f_bytes / f_s / peak, "iter_mpaths_per_s": paths / it_s / 1e6,
+            "iter_algorithmic_bytes": it_bytes, "iter_hbm_roofline_frac": it_bytes / it_s / peak,
             "nonfinite_pixel_fraction": float((~torch.isfinite(target).all(dim=1)).float().mean())}
 
 
