@@ -85,8 +85,17 @@ struct ChainSampler {
         large = cur == 0 || lcg(inner) < large_step_prob;
         cur = cur + 1;
     }
-    // EnsureReady (PSSMLT.fs:68-136)
-    __device__ __forceinline__ float ensure(int i) {
+    // The records of samples [i0, i0 + N), read together so that their loads
+    // overlap: the initialised ones (below `init`; EnsureReady visits them in
+    // index order, so `init` only grows past indices not read here).
+    template <int N>
+    __device__ __forceinline__ void fetch(int i0, float4 (&r)[N]) const {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            r[j] = !FRESH && i0 + j < init ? M->xs[(size_t)(i0 + j) * M->cap + c] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // EnsureReady (PSSMLT.fs:68-136) of sample i, whose record `r` was fetched.
+    __device__ __forceinline__ float ensure(int i, float4 r) {
         if (FRESH) return lcg(inner);
         float4* rec = M->xs + (size_t)i * M->cap + c;
         float v, vb;
@@ -95,7 +104,6 @@ struct ChainSampler {
             v = 0.0f; vb = 0.0f; lm = 0; mb = 0;
             init = i + 1;
         } else {
-            const float4 r = *rec;
             v = r.x; vb = r.y; lm = __float_as_int(r.z); mb = __float_as_int(r.w);
         }
         if (lm < last_large) { v = lcg(inner); lm = last_large; }
@@ -167,14 +175,16 @@ __global__ __launch_bounds__(256) void k_mlt_start(DevScene S, PathBuf P, MltBuf
                 smp.load(M, (uint32_t)i);
                 smp.start_iteration(p.large_step_prob);
             }
-            const float u0 = smp.ensure(0), u1 = smp.ensure(1);
+            float4 rs[4];
+            smp.fetch(0, rs);
+            const float u0 = smp.ensure(0, rs[0]), u1 = smp.ensure(1, rs[1]);
             const float upx = u0 * (float)p.width, upy = u1 * (float)p.height;
             const int ix = min(p.width - 1, sat_int(upx));
             const int iy = min(p.height - 1, sat_int(upy));
             V2 frac = {upx - (float)ix, upy - (float)iy};
             V2 uLens;
-            uLens.x = smp.ensure(2);
-            uLens.y = smp.ensure(3);
+            uLens.x = smp.ensure(2, rs[2]);
+            uLens.y = smp.ensure(3, rs[3]);
             const Ray r = camera_ray(S, ix, iy, frac, uLens);
             P.o[i] = f4(r.o, 0.0f);
             P.d[i] = f4(r.d, 0.0f);
@@ -224,15 +234,17 @@ __global__ __launch_bounds__(256) void k_shade_mlt(DevScene S, PathBuf P, MltBuf
             ChainSampler<FRESH> smp;
             smp.load(M, slot);
             const int s0 = 4 + 7 * depth;
-            const float uLight = smp.ensure(s0);
+            float4 rs[7];
+            smp.fetch(s0, rs);
+            const float uLight = smp.ensure(s0, rs[0]);
             V2 uEmit;
-            uEmit.x = smp.ensure(s0 + 1);
-            uEmit.y = smp.ensure(s0 + 2);
-            const float uLobe = smp.ensure(s0 + 3);
+            uEmit.x = smp.ensure(s0 + 1, rs[1]);
+            uEmit.y = smp.ensure(s0 + 2, rs[2]);
+            const float uLobe = smp.ensure(s0 + 3, rs[3]);
             V2 uBSDF;
-            uBSDF.x = smp.ensure(s0 + 4);
-            uBSDF.y = smp.ensure(s0 + 5);
-            const float uRR = smp.ensure(s0 + 6);
+            uBSDF.x = smp.ensure(s0 + 4, rs[4]);
+            uBSDF.y = smp.ensure(s0 + 5, rs[5]);
+            const float uRR = smp.ensure(s0 + 6, rs[6]);
             smp.store();
             if (info.z >= 0) {
                 const Mat m = load_mat(S, info.z);
@@ -304,6 +316,13 @@ __device__ __forceinline__ void film_add(float* film, const MltParamsDev& p, int
 }
 
 // Metropolis acceptance of one mutation (PSSMLT.fs:347-373).
+// PrimarySample.Restore (PSSMLT.fs:31-33): Value = ValueBackUp,
+// LastModification = ModificationBackUp.
+__device__ __forceinline__ void restore_sample(float4* rec, float4 r) {
+    if (__float_as_uint(r.x) != __float_as_uint(r.y) || __float_as_uint(r.z) != __float_as_uint(r.w))
+        *rec = make_float4(r.y, r.y, r.w, r.w);
+}
+
 __global__ void k_mlt_accept(PathBuf P, MltBuf M, MltParamsDev p, float* film, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -328,12 +347,20 @@ __global__ void k_mlt_accept(PathBuf P, MltBuf M, MltParamsDev p, float* film, i
         if (large) st.y = st.x;                              // Accept()
     } else {
         if (a > 0.0f) film_add(film, p, M.pix_new[i], (w_new * p.inv_eff) * Ln);
-        // Reject(): restore every initialised sample, step the iteration back
-        for (int k = 0; k < st.z; ++k) {
-            float4* rec = M.xs + (size_t)k * M.cap + i;
-            const float4 r = *rec;
-            *rec = make_float4(r.y, r.y, r.w, r.w);
+        // Reject(): restore every initialised sample, step the iteration back.
+        // Eight records in flight per lane; a record that already equals its
+        // backup is left unwritten.
+        float4* rec = M.xs + i;
+        const size_t stride = M.cap;
+        int k = 0;
+        for (; k + 8 <= st.z; k += 8) {
+            float4 r[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] = rec[(size_t)(k + j) * stride];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) restore_sample(rec + (size_t)(k + j) * stride, r[j]);
         }
+        for (; k < st.z; ++k) restore_sample(rec + (size_t)k * stride, rec[(size_t)k * stride]);
         st.x = st.x - 1;
     }
     M.outer[i] = outer;

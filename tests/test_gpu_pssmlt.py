@@ -56,3 +56,20 @@ def test_pssmlt_chunked_chains_match(ctx, orc):
     ofilm, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
     assert np.array_equal(gacc, oacc) and gst["b"] == ost["b"]
     del native
+
+
+def test_pssmlt_material_sorted_shading_matches(ctx, orc):
+    """Shading in material-sorted order (material_sort 1) instead of queue
+    order leaves every chain's trajectory unchanged."""
+    sc = caustic(0)
+    p = sc.mlt_params()
+    ds = ctx.scene(sc)
+    ctx.set_option("material_sort", 1)
+    try:
+        _, gst = ds.render_pssmlt(p)
+        gboot, gacc = ds.pssmlt_last(p)
+    finally:
+        ctx.set_option("material_sort", -1)
+    _, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
+    assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
+    assert np.array_equal(gacc, oacc) and gst["b"] == ost["b"]
