@@ -272,6 +272,37 @@ def caustic_scene(width=1024, height=1024, spp=256, max_depth=16, freq=32,
     return info
 
 
+def sphere_scene(width=64, height=64, spp=4, max_depth=8):
+    """The scene features of the reference's own asset that configs 0-4 leave
+    out: analytic spheres (SpherePrimitive; uniform and non-uniform scale),
+    a sphere light (SphereInstance.Sample / EvalPDF) beside the quad light, a
+    rotated mirror cube and a thin-lens camera with a finite aperture."""
+    info = cornell_box(width, height, spp, max_depth, boxes=False)
+    prims, mats, insts, tfs, nodes = (info[k] for k in ("primitives", "materials", "instances", "transforms", "nodes"))
+    cam_node = nodes.pop()                           # keep the camera node last
+    prims.append({"type": "sphere", "radius": 1.0})
+    prims.append({"type": "cube"})
+    sphere, cube = len(prims) - 2, len(prims) - 1
+    mats += [{"type": "pbr", "albedo": [0.9, 0.85, 0.8], "metallic": 1.0, "roughness": 0.2},
+             {"type": "dielectric", "ior": 1.5},
+             {"type": "mirror", "albedo": [0.95, 0.95, 0.95]}]
+    info["lights"].append({"type": "diffuse", "emission": [4.0, 4.0, 8.0]})
+    placed = [({"primitive": sphere, "material": 3}, {"scale": [90.0, 90.0, 90.0], "translation": [150.0, 90.0, -200.0]}),
+              ({"primitive": sphere, "material": 4}, {"scale": [70.0, 110.0, 70.0], "translation": [400.0, 110.0, -300.0]}),
+              ({"primitive": sphere, "light": 1}, {"scale": [30.0, 30.0, 30.0], "translation": [280.0, 420.0, -150.0]}),
+              ({"primitive": cube, "material": 5}, {"scale": [50.0, 50.0, 50.0], "rotation": [0.3, 0.5, 0.0],
+                                                     "translation": [300.0, 50.0, -100.0]})]
+    for inst, tf in placed:
+        insts.append(inst)
+        tfs.append({"keyframes": [dict(time=0.0, **tf)]})
+        nodes.append({"instances": [len(insts) - 1], "transform": len(tfs) - 1})
+    nodes.append(cam_node)
+    nodes[0]["children"] = list(range(1, len(nodes)))
+    info["camera"] = {"type": "thin-lens", "fov": 39.3, "aspect-ratio": 1.0, "aperture": 12.0,
+                      "focus-distance": 1000.0, "push-forward": 0.0}
+    return info
+
+
 def batch_scene(seed: int, max_res: int = 1024, max_spp: int = 64, tri_range=(1000, 500000),
                 max_instances: int = 256):
     """One scene of the 32-scene batch: a ground quad, a pool of at most 8

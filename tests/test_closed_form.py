@@ -69,3 +69,53 @@ def test_nee_only_depth_one_is_biased_low(orc):
     se = vals.std() / np.sqrt(len(vals))
     assert vals.mean() + 4 * se < 0.97 * ref
     assert vals.mean() > 0.5 * ref
+
+
+def sphere_instances(sc, osc):
+    """Internal instance index -> world-to-object matrix (row-vector
+    System.Numerics layout) of every analytic-sphere instance."""
+    ex = osc.export()
+    out = {}
+    for k, i in enumerate(ex["inst_order"]):
+        if sc.primitives[sc.instances[i][0]]["kind"] == 1:
+            out[k] = ex["w2o"][k].astype(np.float64).reshape(4, 4)
+    return out
+
+
+def test_sphere_hits_match_closed_form(orc):
+    """SpherePrimitive.Intersect (Sphere.fs:35-77) under uniform and
+    non-uniform instance scale, with thin-lens camera rays: where the oracle
+    reports a sphere hit, its t equals the first root of |o' + t d'| = 1 for
+    the object-space ray (o', d') of that instance (solved in double); and
+    wherever such a root exists, the oracle's hit is not beyond it.  A wrong
+    root, a dropped term or an inverted scale fails both.  The tolerance,
+    1e-4 of t, covers the float32 transform of the ray to object space; the
+    other root lies 2 sqrt(disc) / a away.  Near-tangent rays (discriminant
+    below 1e-3 of b^2) are left out: float32 may miss them."""
+    from barnacle_amd import configs
+    sc = scene.load_scene(configs.sphere_scene(48, 48, 1, 8))
+    osc = orc.scene(sc)
+    p = sc.pt_params()
+    inst, _, t = osc.first_hit(p, 0)
+    spheres = sphere_instances(sc, osc)
+    assert len(spheres) == 3
+    n_hits = 0
+    for y in range(p.height):
+        for x in range(p.width):
+            o, d = osc.camera_ray(p, x, y, 0)
+            k = y * p.width + x
+            for i, m in spheres.items():
+                oo = np.append(o.astype(np.float64), 1.0) @ m
+                dd = np.append(d.astype(np.float64), 0.0) @ m
+                a, b, cc = dd[:3] @ dd[:3], oo[:3] @ dd[:3], oo[:3] @ oo[:3] - 1.0
+                disc = b * b - a * cc
+                if disc < 1e-3 * b * b:
+                    continue
+                roots = [r for r in ((-b - np.sqrt(disc)) / a, (-b + np.sqrt(disc)) / a) if r > 1e-3]
+                if not roots:
+                    continue
+                assert t[k] <= roots[0] * (1 + 1e-4)
+                if inst[k] == i:
+                    assert abs(t[k] - roots[0]) <= 1e-4 * roots[0]
+                    n_hits += 1
+    assert n_hits > 100
