@@ -72,3 +72,19 @@ def test_batch_with_spheres_equals_plain(ctx, sph):
     films, _ = ctx.render_batch(jobs)
     for (ds, p), f in zip(jobs, films):
         assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
+
+
+def test_nested_node_transforms_first_hit(ctx, orc):
+    """A sphere two node levels down (transforms composed local * parent):
+    the GPU scene build places it where the oracle does, whose placement
+    tests/test_closed_form.py pins to the closed form."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_closed_form import nested_sphere_scene
+    sc = scene.load_scene(nested_sphere_scene(48, 48))
+    p = sc.pt_params()
+    gi, gp, gt = ctx.scene(sc).first_hit(p, 0)
+    oi, op, ot = orc.scene(sc).first_hit(p, 0)
+    assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+    assert (gi >= 0).sum() > 100
