@@ -276,13 +276,15 @@ def sphere_scene(width=64, height=64, spp=4, max_depth=8):
     """The scene features of the reference's own asset that configs 0-4 leave
     out: analytic spheres (SpherePrimitive; uniform and non-uniform scale),
     a sphere light (SphereInstance.Sample / EvalPDF) beside the quad light, a
-    rotated mirror cube and a thin-lens camera with a finite aperture."""
+    rotated mirror cube, a panel of the builtin quad and a thin-lens camera
+    with a finite aperture."""
     info = cornell_box(width, height, spp, max_depth, boxes=False)
     prims, mats, insts, tfs, nodes = (info[k] for k in ("primitives", "materials", "instances", "transforms", "nodes"))
     cam_node = nodes.pop()                           # keep the camera node last
     prims.append({"type": "sphere", "radius": 1.0})
     prims.append({"type": "cube"})
-    sphere, cube = len(prims) - 2, len(prims) - 1
+    prims.append({"type": "quad"})
+    sphere, cube, quad = len(prims) - 3, len(prims) - 2, len(prims) - 1
     mats += [{"type": "pbr", "albedo": [0.9, 0.85, 0.8], "metallic": 1.0, "roughness": 0.2},
              {"type": "dielectric", "ior": 1.5},
              {"type": "mirror", "albedo": [0.95, 0.95, 0.95]}]
@@ -291,7 +293,9 @@ def sphere_scene(width=64, height=64, spp=4, max_depth=8):
               ({"primitive": sphere, "material": 4}, {"scale": [70.0, 110.0, 70.0], "translation": [400.0, 110.0, -300.0]}),
               ({"primitive": sphere, "light": 1}, {"scale": [30.0, 30.0, 30.0], "translation": [280.0, 420.0, -150.0]}),
               ({"primitive": cube, "material": 5}, {"scale": [50.0, 50.0, 50.0], "rotation": [0.3, 0.5, 0.0],
-                                                     "translation": [300.0, 50.0, -100.0]})]
+                                                     "translation": [300.0, 50.0, -100.0]}),
+              ({"primitive": quad, "material": 0}, {"scale": [80.0, 1.0, 60.0], "rotation": [1.5707964, 0.0, 0.2],
+                                                     "translation": [120.0, 330.0, -500.0]})]
     for inst, tf in placed:
         insts.append(inst)
         tfs.append({"keyframes": [dict(time=0.0, **tf)]})
