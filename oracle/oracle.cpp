@@ -1458,7 +1458,9 @@ void liPTGrad(const orc_scene& s, Ray ray, Sampler& smp, int maxDepth, int rrDep
 
 int orc_render(orc_scene* s, int32_t integ, const orc_pt_params* p, int32_t n_threads, float* film,
                orc_stats* stats, int32_t x0, int32_t y0, int32_t x1, int32_t y1) {
-    if (s->lights.empty() && integ != 0) return fail("LightSamplerBase: No light primitives found.");
+    // Scene.Render builds the UniformLightSampler before any integrator runs
+    // (Extensions/Scene/Render.fs:13-14), so the normal view fails as well.
+    if (s->lights.empty()) return fail("LightSamplerBase: No light primitives found.");
     if (x1 <= x0 || y1 <= y0) { x0 = 0; y0 = 0; x1 = p->width; y1 = p->height; }
     const int TS = 16;                                      // Integrator.fs:16
     int tx = (p->width + TS - 1) / TS, ty = (p->height + TS - 1) / TS;

@@ -1,0 +1,56 @@
+"""Failure handling on the GPU path, forced from the host (no kernel is made
+to fault): a scene without a light is refused with the reference's message
+(Base/LightSampler.fs:8-9) before anything launches, for every integrator
+(Scene.Render builds the light sampler first, Render.fs:13-14); a PSSMLT scene whose lights emit nothing has
+B = 0 and is skipped with a black film (PSSMLT.fs:396-397), as the oracle
+does; invalid parameters return BN_E_INVALID."""
+import copy
+
+import numpy as np
+import pytest
+
+from barnacle_amd import abi, configs, native, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def test_scene_without_light_is_refused(ctx):
+    info = configs.cornell_box(16, 16, 1, 2)
+    info["instances"] = [{"primitive": i["primitive"], "material": 0} if "light" in i else i
+                         for i in info["instances"]]
+    sc = scene.load_scene(info)
+    ds = ctx.scene(sc)
+    with pytest.raises(native.BarnacleError, match="No light primitives found") as e:
+        ds.render_pt(sc.pt_params())
+    assert e.value.code == abi.BN_E_NO_LIGHT
+    with pytest.raises(native.BarnacleError, match="No light primitives found"):
+        ctx.render_batch([(ds, sc.pt_params())])
+    with pytest.raises(native.BarnacleError, match="No light primitives found"):
+        ds.render_normal(sc.pt_params())
+    # the context stays usable
+    ok = scene.load_scene(configs.cornell_box(16, 16, 1, 2))
+    assert np.isfinite(ctx.scene(ok).render_pt(ok.pt_params())).all()
+
+
+def test_pssmlt_black_scene_is_skipped(ctx, orc):
+    info = configs.caustic_scene(16, 16, 4, 4, freq=4, n_bootstrap=256, n_chains=32)
+    info["lights"] = [{"type": "diffuse", "emission": [0.0, 0.0, 0.0]}]
+    sc = scene.load_scene(info)
+    p = sc.mlt_params()
+    film, st = ctx.scene(sc).render_pssmlt(p)
+    ofilm, ost, _, _ = orc.scene(sc).render_pssmlt(p)
+    assert st["b"] == ost["b"] == 0.0
+    assert st["proposed"] == ost["proposed"] == 0
+    assert np.array_equal(film, ofilm) and not film.any()
+
+
+def test_invalid_parameters(ctx):
+    sc = scene.load_scene(configs.cornell_box(16, 16, 1, 2))
+    ds = ctx.scene(sc)
+    for bad in (dict(width=0), dict(spp=-1), dict(max_depth=-1)):
+        with pytest.raises(native.BarnacleError) as e:
+            ds.render_pt(sc.pt_params(**bad))
+        assert e.value.code == abi.BN_E_INVALID
+    with pytest.raises(native.BarnacleError) as e:
+        ctx.set_option("stack_k", 24)
+    assert e.value.code == abi.BN_E_INVALID
