@@ -88,3 +88,13 @@ def test_nested_node_transforms_first_hit(ctx, orc):
     assert np.array_equal(gi, oi) and np.array_equal(gp, op)
     assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
     assert (gi >= 0).sum() > 100
+
+
+def test_normal_view_spheres(ctx, orc, sph):
+    """NormalIntegrator (Extensions/Integrator/Normal.fs:10-17) on the sphere
+    scene: sphere normals (Sphere.fs:60-77) through instance transforms."""
+    p = sph.pt_params(spp=2)
+    g = ctx.scene(sph).render_normal(p)
+    o, _ = orc.scene(sph).render(p, integrator=0)
+    assert np.mean(g == o) > 0.999
+    assert np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)) <= 1e-5
