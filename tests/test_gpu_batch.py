@@ -92,3 +92,23 @@ def test_batch_scene_rounds_equal_plain(ctx, jobs, graph):
     assert (st["rounds"], st["scene_rounds"], st["graph"]) == (3, 2, graph)
     for (ds, p), f in zip(work, films):
         assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
+
+
+@pytest.mark.parametrize("seed", list(range(6, 16)))
+def test_more_batch_scenes_match_oracle(ctx, orc, seed):
+    """Ten more seeds of the 32-scene generator at 32x32, 2 spp: first hits
+    bit-exact and films within rel-RMSE 1e-3 of the oracle."""
+    sc = scene.load_scene(configs.batch_scene(seed, max_res=32, max_spp=2, tri_range=(200, 3000),
+                                              max_instances=16))
+    p = sc.pt_params()
+    ds = ctx.scene(sc)
+    gi, gp, gt = ds.first_hit(p, 0)
+    oi, op, ot = orc.scene(sc).first_hit(p, 0)
+    assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+    g = ds.render_pt(p)
+    o, _ = orc.scene(sc).render(p)
+    ok = np.all(np.isfinite(o), axis=1)
+    assert np.array_equal(np.all(np.isfinite(g), axis=1), ok)
+    rel = np.sqrt(np.sum((g[ok] - o[ok]) ** 2) / max(np.sum(o[ok] ** 2), 1e-30))
+    assert rel <= 1e-3
