@@ -224,7 +224,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        print("bench.py measures one MI355X: the renderer has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     sc = scene.load_scene(configs.config1(spp=args.spp))
