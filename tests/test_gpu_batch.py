@@ -112,3 +112,20 @@ def test_more_batch_scenes_match_oracle(ctx, orc, seed):
     assert np.array_equal(np.all(np.isfinite(g), axis=1), ok)
     rel = np.sqrt(np.sum((g[ok] - o[ok]) ** 2) / max(np.sum(o[ok] ** 2), 1e-30))
     assert rel <= 1e-3
+
+
+def test_batch_graph_cache_overflow(ctx):
+    """More rounds over distinct scenes than the context keeps captured
+    graphs (64): the cache is cleared between rounds and refilled, and every
+    film still equals its plain render."""
+    info = configs.cornell_box(8, 8, 2, 4)
+    scs = [scene.load_scene(info) for _ in range(70)]
+    jobs = [(ctx.scene(sc), sc.pt_params(frame_id=k)) for k, sc in enumerate(scs)]
+    ctx.set_option("window_paths", 256)
+    try:
+        films, st = ctx.render_batch(jobs)
+    finally:
+        ctx.set_option("window_paths", 0)
+    assert st["rounds"] == 70 and st["scene_rounds"] == 70 and st["graph"] == 1
+    for (ds, p), f in zip(jobs, films):
+        assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
