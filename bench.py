@@ -51,8 +51,10 @@ def frame_bytes(st):
     return closest_bytes(st) + shadow + 192 * st["vertices"] + 12 * st["paths"]
 
 
-def cpu_baseline(sc, params, budget_s=12.0):
-    """Oracle rays/s on 64x64 pixel windows of the same frame, centre outwards."""
+def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
+    """Oracle rays/s on 64x64 pixel windows of the same frame, centre outwards.
+    With the GPU's film of that frame, the windows also give a full-size
+    parity check: rel-RMSE and the bit-equal fraction over their pixels."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = min(16, os.cpu_count() or 1)
@@ -61,20 +63,35 @@ def cpu_baseline(sc, params, budget_s=12.0):
     tiles = sorted(((x, y) for y in range(0, H, 64) for x in range(0, W, 64)),
                    key=lambda t: (t[0] + 32 - W / 2) ** 2 + (t[1] + 32 - H / 2) ** 2)
     rays, paths, elapsed, n = 0, 0, 0.0, 0
+    got, ref = [], []
     for (x, y) in tiles:
+        x1, y1 = min(x + 64, W), min(y + 64, H)
         t0 = time.perf_counter()
-        _, st = osc.render(params, threads=threads, window=(x, y, min(x + 64, W), min(y + 64, H)))
+        film, st = osc.render(params, threads=threads, window=(x, y, x1, y1))
         elapsed += time.perf_counter() - t0
+        if gpu_film is not None:
+            # Film.SetPixel stores row y at H - y - 1 (Film.fs:43)
+            rows = np.arange(H - y1, H - y)[:, None] * W + np.arange(x, x1)[None, :]
+            got.append(gpu_film[rows.reshape(-1)])
+            ref.append(film[rows.reshape(-1)])
         rays += st["rays_closest"] + st["rays_shadow"]
         paths += st["paths"]
         n += 1
         if elapsed >= budget_s:
             break
-    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{n} 64x64-pixel windows at the film centre of the same 1024x1024x64spp frame "
-                      f"({paths} paths, {rays} rays, {elapsed:.1f} s); oracle = C++ restatement of the F# "
-                      f"renderer (no .NET runtime available)",
-            "mpaths_per_s": paths / elapsed / 1e6}
+    out = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"{n} 64x64-pixel windows at the film centre of the same 1024x1024x64spp frame "
+                     f"({paths} paths, {rays} rays, {elapsed:.1f} s); oracle = C++ restatement of the F# "
+                     f"renderer (no .NET runtime available)",
+           "mpaths_per_s": paths / elapsed / 1e6}
+    if got:
+        g, o = np.concatenate(got), np.concatenate(ref)
+        ok = np.isfinite(o).all(axis=1)
+        out["frame_parity"] = {"pixels": int(len(o)),
+                               "rel_rmse": float(np.sqrt(np.sum((g[ok] - o[ok]) ** 2) / np.sum(o[ok] ** 2))),
+                               "bit_equal_fraction": float(np.mean(g.view(np.int32) == o.view(np.int32))),
+                               "nonfinite_masks_equal": bool(np.array_equal(np.isfinite(g), np.isfinite(o)))}
+    return out
 
 
 def batched(ctx, n=32):
@@ -244,8 +261,9 @@ def main():
     ctx.set_option("timing", 1)
     ctx.reset_kernel_times()
     t0 = time.perf_counter()
+    film = None
     for i in range(args.steps):
-        ds.render_pt(params)
+        film = ds.render_pt(params)
         print(f"# step {i} done", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     kt = ctx.kernel_times()
@@ -301,7 +319,7 @@ def main():
     if args.copy_calibration:
         copy_calibration()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, params)
+        out["cpu_baseline"] = cpu_baseline(sc, params, film)
     print(json.dumps(out), flush=True)
 
 
