@@ -223,8 +223,27 @@ def parity(ctx):
     g = ctx.scene(sc).render_pt(p)
     o, _ = oracle.Oracle().scene(sc).render(p, threads=min(16, os.cpu_count() or 1))
     rel = float(np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)))
-    return {"config": "same scene at 128x128x2spp", "rel_rmse": rel, "bit_equal_fraction": float(np.mean(g == o)),
-            "tolerance": 1e-3}
+    out = {"config": "same scene at 128x128x2spp", "rel_rmse": rel, "bit_equal_fraction": float(np.mean(g == o)),
+           "tolerance": 1e-3}
+    # config 0, the reference's CPU-runnable case, whole frame on both sides
+    sc0 = scene.load_scene(configs.config0())
+    p0 = sc0.pt_params()
+    ds0 = ctx.scene(sc0)
+    ds0.render_pt(p0)
+    t0 = time.perf_counter()
+    g0 = ds0.render_pt(p0)
+    gpu_s = time.perf_counter() - t0
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    o0, st0 = oracle.Oracle().scene(sc0).render(p0, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    out["config0"] = {"config": "config0: Cornell box, 256x256, 16 spp, depth 8, whole frame",
+                      "gpu_ms": gpu_s * 1e3, "oracle_s": cpu_s, "oracle_threads": threads,
+                      "speedup": cpu_s / gpu_s,
+                      "gpu_mrays_per_s": (st0["rays_closest"] + st0["rays_shadow"]) / gpu_s / 1e6,
+                      "rel_rmse": float(np.sqrt(np.sum((g0 - o0) ** 2) / np.sum(o0 ** 2))),
+                      "bit_equal_fraction": float(np.mean(g0.view(np.int32) == o0.view(np.int32)))}
+    return out
 
 
 def main():
