@@ -273,7 +273,9 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
  * a scene from bn_scene_build / bn_upload_scene; all items advance through
  * one launch sequence per round (scenes, resolutions, spp and depths may
  * differ), with a block of iterations captured once as a HIP graph and
- * replayed.  films[k] receives item k's film exactly as bn_render_pt would
+ * replayed.  An item with at least an eighth of a round of paths gets rounds
+ * over its own scene (with that scene's other items); smaller items share
+ * rounds.  films[k] receives item k's film exactly as bn_render_pt would
  * write it (bit-identical); NULL entries are skipped. */
 typedef struct bn_batch_item {
     int32_t      scene;
@@ -299,7 +301,8 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
  *   "stack_k"              LDS window of the traversal stack: 16 (default) or 32
  *   "graph"                1 (default): replay batched rounds as a HIP graph
- *   "batch_pack"           0 (default): pack rounds item-major; 1: fair share
+ *   "batch_pack"           0 (default): pack rounds item-major, rounds over one
+ *                          scene for big items; 1: fair share
  *   "material_sort"        1: bin the shading queue by material kind; 0: queue
  *                          order, which keeps path-state access coalesced;
  *                          -1 (default): bins for gradient rounds only, whose
