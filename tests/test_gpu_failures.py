@@ -54,3 +54,27 @@ def test_invalid_parameters(ctx):
     with pytest.raises(native.BarnacleError) as e:
         ctx.set_option("stack_k", 24)
     assert e.value.code == abi.BN_E_INVALID
+
+
+def test_two_contexts_interleaved(ctx):
+    """A second context on the same GPU, with its own scenes, buffers, graphs
+    and options, interleaved with the first: both keep producing the films
+    they produce alone."""
+    sc = scene.load_scene(configs.config1(width=48, height=40, spp=2, max_depth=8, freq=12))
+    p = sc.pt_params()
+    a = ctx.scene(sc)
+    ref = a.render_pt(p)
+    other = native.Context(0)
+    try:
+        other.set_option("stack_k", 32)
+        other.set_option("trace_blocks_per_cu", 3)
+        b = other.scene(sc)
+        for _ in range(2):
+            fb = b.render_pt(p)
+            fa = a.render_pt(p)
+            films, _ = other.render_batch([(b, p), (b, sc.pt_params(frame_id=1))])
+            assert np.array_equal(fa.view(np.int32), ref.view(np.int32))
+            assert np.array_equal(fb.view(np.int32), ref.view(np.int32))
+            assert np.array_equal(films[0].view(np.int32), ref.view(np.int32))
+    finally:
+        other.close()
