@@ -12,7 +12,8 @@ constexpr int kStackK = 32;       // default LDS window of the per-lane traversa
 constexpr int kStackKMin = 16;    // smallest window (sizes the spill area)
 constexpr int kStackMax = 144;    // TLAS + BLAS depth <= 2 * (64 + 1) entries
 constexpr int kBlock = 256;
-constexpr int kFetchSegs = 64;    // fetch counters per trace queue
+constexpr int kFetchSegs = 128;   // fetch counters per trace queue (closest-hit traversal)
+constexpr int kShadowSegs = 64;   // of which the any-hit traversals use the first 64
 constexpr int kSegStride = 64;    // words between two fetch counters (own cache lines)
 
 constexpr uint32_t kNoPath = 0xffffffffu;

@@ -695,7 +695,7 @@ __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWa
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT, K>(
-        SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        SegFetch{seg, kShadowSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq ? shq[i] : i;
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(256, K == 16 ? kTraceWaves : 1) void k_trace_shadow
     const uint32_t count = ctl->sh_count;
     TraceCount tc{0, 0, 0};
     persistent_trace<true, false, K>(
-        SegFetch{seg, kFetchSegs}, count, stk, spill, spill_stride, refill, tc,
+        SegFetch{seg, kShadowSegs}, count, stk, spill, spill_stride, refill, tc,
         [&](Traversal<K>& tr) { W.bind(tr.G); },
         [&](Traversal<K>& tr, uint32_t i) {
             const uint32_t slot = shq[i];
