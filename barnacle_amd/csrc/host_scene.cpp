@@ -153,8 +153,13 @@ void build_bvh(const std::vector<Box>& in, std::vector<bn_bvh_node>& nodes, std:
             right_first = split_first;
         } else {
             const float lo = cb.lo[axis];
+            // A NaN among the centroids can leave `lo` above other centroids,
+            // so the index can be negative; the reference's bounds-checked
+            // Span indexer throws there (Util/BVH.fs:163-174), and so does this.
             auto bin_of = [&](const Box& b) {
-                return std::min(kBins - 1, sat_int((12.0f * (b.center()[axis] - lo)) / extent));
+                const int k = std::min(kBins - 1, sat_int((12.0f * (b.center()[axis] - lo)) / extent));
+                if (k < 0) throw Error(BN_E_INVALID, "BVH build: bin index out of range (non-finite bounds)");
+                return k;
             };
             Box bb[kBins];
             int bc[kBins] = {};

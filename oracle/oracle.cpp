@@ -266,9 +266,12 @@ void bvhBuild(std::vector<Item>& items, int first, int last, int depth, std::vec
     int bc[NB];
     for (int i = 0; i < NB; ++i) { bb[i] = aabbDefault(); bc[i] = 0; }
     float cmin = comp(cb.mn, axis);
+    // Util/BVH.fs:163-174: a negative index (a NaN centroid can leave cmin
+    // above other centroids) makes the Span indexer throw.
     auto binOf = [&](const Item& it) {
-        int b = satInt((12.0f * (comp(centroid(it.b), axis) - cmin)) / extent);
-        return std::min(NB - 1, b);
+        int b = std::min(NB - 1, satInt((12.0f * (comp(centroid(it.b), axis) - cmin)) / extent));
+        if (b < 0) throw std::runtime_error("BVH build: bin index out of range (non-finite bounds)");
+        return b;
     };
     for (int i = first; i < last; ++i) {
         const Item& it = sub[i - first];

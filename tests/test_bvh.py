@@ -226,3 +226,16 @@ def test_blas_of_config_mesh_matches_oracle(orc):
     assert nodes.tobytes() == ob["nodes"].tobytes()
     assert np.array_equal(I[perm].reshape(-1), ob["tri"])
     check_invariants(nodes, perm, np.concatenate([T.min(axis=1), T.max(axis=1)], axis=1))
+
+
+def test_negative_bin_index_raises_instead_of_writing_out_of_bounds():
+    """Non-finite item bounds can make the binned-SAH index negative (a NaN
+    centroid leaves the running minimum above other centroids); the
+    reference's bounds-checked Span throws there (Util/BVH.fs:163-174), and
+    the host build returns an error instead of writing outside its bins."""
+    import pytest
+    from barnacle_amd import native
+    b = np.array([[0, 0, 0, 1, 1, 1], [2, 0, 0, 3, 1, 1], [4, 0, 0, 5, 1, 1], [np.nan] * 6,
+                  [6, 0, 0, 7, 1, 1], [8, 0, 0, 9, 1, 1], [10, 0, 0, 11, 1, 1]], np.float32)
+    with pytest.raises(native.BarnacleError, match="bin index out of range"):
+        native.bvh_build(b)

@@ -91,3 +91,49 @@ def test_pt_shallow_depths(ctx, orc, c0, depth):
         assert not np.any(g)
     assert np.mean(g == o) > 0.999
     assert rel_rmse(g, o) <= 1e-3 if np.any(o) else not np.any(g)
+
+
+def nonfinite_scene(vertices):
+    info = configs.cornell_box(48, 48, 2, 4)
+    info["primitives"].append({"type": "mesh", "vertices": np.array(vertices, np.float32).reshape(-1),
+                               "indices": np.arange(len(vertices))})
+    info["instances"].append({"primitive": len(info["primitives"]) - 1, "material": 0})
+    info["nodes"][0]["instances"].append(len(info["instances"]) - 1)
+    return scene.load_scene(info)
+
+
+def test_nonfinite_geometry_matches_oracle(ctx, orc):
+    """A triangle with an infinite vertex: its bounds, and the TLAS bounds
+    above it, are infinite, so the scene traverses with the exact
+    NaN-propagating slab test (Ray.fs:26-39) everywhere, and about half of
+    the camera rays miss, as in the reference.  First hits, counters and film
+    as the oracle's."""
+    sc = nonfinite_scene([[100, 100, -300], [200, 100, -300], [np.inf, 200, -300]])
+    p = sc.pt_params()
+    ds, os_ = ctx.scene(sc), orc.scene(sc)
+    gi, gp, gt = ds.first_hit(p, 0)
+    oi, op, ot = os_.first_hit(p, 0)
+    assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+    assert (oi < 0).sum() > 500 and (oi >= 0).sum() > 500
+    g, gst = ds.render_pt(p, stats=True)
+    o, ost = os_.render(p)
+    for k in ("nodes_closest", "tris_closest", "insts_closest", "nodes_shadow", "tris_shadow"):
+        assert gst[k] == ost[k], k
+    assert np.array_equal(np.isfinite(g), np.isfinite(o))
+    ok = np.all(np.isfinite(o), axis=1)
+    assert rel_rmse(g[ok], o[ok]) <= 1e-3
+
+
+def test_nan_bounds_fail_the_build_like_the_reference(ctx, orc):
+    """A NaN vertex makes a TLAS bin index negative (Util/BVH.fs:163-174),
+    where the reference's Span indexer throws: the scene build fails with an
+    error on both sides, and the context stays usable."""
+    from barnacle_amd import native
+    sc = nonfinite_scene([[300, 300, -250], [400, 300, -250], [350, np.nan, -250]])
+    with pytest.raises(native.BarnacleError, match="bin index out of range"):
+        ctx.scene(sc)
+    with pytest.raises(RuntimeError, match="bin index out of range"):
+        orc.scene(sc)
+    ok = scene.load_scene(configs.cornell_box(16, 16, 1, 2))
+    assert np.isfinite(ctx.scene(ok).render_pt(ok.pt_params())).all()
