@@ -137,3 +137,26 @@ def test_nan_bounds_fail_the_build_like_the_reference(ctx, orc):
         orc.scene(sc)
     ok = scene.load_scene(configs.cornell_box(16, 16, 1, 2))
     assert np.isfinite(ctx.scene(ok).render_pt(ok.pt_params())).all()
+
+
+def test_singular_instance_transform_matches_oracle(ctx, orc):
+    """A box flattened to zero height: Matrix4x4.Invert of its transform is
+    singular, so its world-to-object map is not finite and rays entering it
+    take the exact path.  First hits, counters and film as the oracle's."""
+    info = configs.cornell_box(48, 48, 2, 4, boxes=True)
+    info["transforms"][0]["keyframes"][0]["scale"] = [82.5, 0.0, 82.5]
+    sc = scene.load_scene(info)
+    p = sc.pt_params()
+    ds, os_ = ctx.scene(sc), orc.scene(sc)
+    assert not np.isfinite(os_.export()["w2o"]).all()
+    gi, gp, gt = ds.first_hit(p, 0)
+    oi, op, ot = os_.first_hit(p, 0)
+    assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+    g, gst = ds.render_pt(p, stats=True)
+    o, ost = os_.render(p)
+    for k in ("nodes_closest", "tris_closest", "insts_closest", "nodes_shadow", "tris_shadow"):
+        assert gst[k] == ost[k], k
+    assert np.array_equal(np.isfinite(g), np.isfinite(o))
+    ok = np.all(np.isfinite(o), axis=1)
+    assert rel_rmse(g[ok], o[ok]) <= 1e-3
