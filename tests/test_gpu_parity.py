@@ -160,3 +160,18 @@ def test_singular_instance_transform_matches_oracle(ctx, orc):
     assert np.array_equal(np.isfinite(g), np.isfinite(o))
     ok = np.all(np.isfinite(o), axis=1)
     assert rel_rmse(g[ok], o[ok]) <= 1e-3
+
+
+@pytest.mark.parametrize("w,h,spp", [(37, 23, 1), (1, 1, 3), (13, 1, 2), (9, 9, 0)])
+def test_ragged_and_empty_films(ctx, orc, w, h, spp):
+    """Films whose sides are not multiples of the 8x8 path tiles, a single
+    pixel, a single row, and zero samples per pixel (an all-zero film, as
+    ProgressiveIntegrator leaves it): the oracle's film."""
+    sc = scene.load_scene(configs.config0())
+    p = sc.pt_params(width=w, height=h, spp=spp)
+    g = ctx.scene(sc).render_pt(p)
+    o, _ = orc.scene(sc).render(p)
+    assert g.shape == (w * h, 3)
+    assert np.array_equal(g.view(np.int32), o.view(np.int32))
+    if spp == 0:
+        assert not g.any()
