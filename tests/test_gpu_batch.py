@@ -129,3 +129,20 @@ def test_batch_graph_cache_overflow(ctx):
     assert st["rounds"] == 70 and st["scene_rounds"] == 70 and st["graph"] == 1
     for (ds, p), f in zip(jobs, films):
         assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
+
+
+def test_batch_ragged_and_empty_items(ctx):
+    """A batch of a single pixel, a zero-sample item, a single row and a
+    ragged film: every film equals its plain render."""
+    sc = scene.load_scene(configs.config0())
+    ds = ctx.scene(sc)
+    jobs = [(ds, sc.pt_params(width=w, height=h, spp=s, frame_id=k))
+            for k, (w, h, s) in enumerate([(1, 1, 3), (9, 9, 0), (13, 1, 2), (37, 23, 1)])]
+    for graph in (1, 0):
+        ctx.set_option("graph", graph)
+        try:
+            films, _ = ctx.render_batch(jobs)
+        finally:
+            ctx.set_option("graph", 1)
+        for (d, p), f in zip(jobs, films):
+            assert np.array_equal(f.view(np.int32), d.render_pt(p).view(np.int32))
