@@ -73,3 +73,20 @@ def test_pssmlt_material_sorted_shading_matches(ctx, orc):
     _, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
     assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
     assert np.array_equal(gacc, oacc) and gst["b"] == ost["b"]
+
+
+@pytest.mark.parametrize("n_bootstrap,n_chains", [(7, 3), (1, 1), (300, 257)])
+def test_pssmlt_odd_counts_match_oracle(ctx, orc, n_bootstrap, n_chains):
+    """Bootstrap and chain counts that fill no wave or block evenly, down to
+    one of each: bootstrap, B, per-chain accept counts as the oracle's."""
+    info = configs.caustic_scene(width=16, height=16, spp=4, max_depth=6, freq=6,
+                                 n_bootstrap=n_bootstrap, n_chains=n_chains, sigma=1.0 / 64.0, large_step=0.3)
+    sc = scene.load_scene(info)
+    p = sc.mlt_params()
+    ds = ctx.scene(sc)
+    _, gst = ds.render_pssmlt(p)
+    gboot, gacc = ds.pssmlt_last(p)
+    _, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
+    assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
+    assert gst["b"] == ost["b"] and gst["mutations_per_chain"] == ost["mutations_per_chain"]
+    assert np.array_equal(gacc, oacc)
