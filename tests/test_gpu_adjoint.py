@@ -128,3 +128,22 @@ def test_adjoint_sphere_scene_matches_fp64_oracle(ctx, orc):
         assert scale > 0
         assert np.allclose(g, o, rtol=1e-2, atol=2e-3 * scale)
         assert np.linalg.norm(g - o) <= 5e-3 * np.linalg.norm(o)
+
+
+def test_adjoint_lambertian_box_and_zero_loss(ctx, orc):
+    """The plain Cornell box (three Lambertian materials, one quad light):
+    gradients as the FP64 oracle's; a zero film adjoint gives exactly zero
+    gradients."""
+    sc = scene.load_scene(configs.cornell_box(24, 24, 4, 5))
+    p = sc.pt_params()
+    ds = ctx.scene(sc)
+    film = ds.render_pt(p)
+    d = film_adjoint(film)
+    gm, gl = ds.backward_pt(p, d)
+    om, ol = orc.scene(sc).adjoint_pt(p, d)
+    for g, o in ((gm[:, :3], om[:, :3]), (gl, ol)):
+        scale = np.abs(o).max()
+        assert scale > 0
+        assert np.allclose(g, o, rtol=1e-2, atol=2e-3 * scale)
+    zm, zl = ds.backward_pt(p, np.zeros_like(d))
+    assert not zm.any() and not zl.any()
