@@ -1063,6 +1063,7 @@ int bn_create(int device, bn_ctx** out) {
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->n_cu = prop.multiProcessorCount;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        noalloc_init(device);
         *out = reinterpret_cast<bn_ctx*>(c.release());
     });
 }
@@ -1288,6 +1289,17 @@ int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* p, int32_t samp
         hip_check(hipMemcpyAsync(prim, c->out_b.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipMemcpyAsync(t, c->out_c.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "first hit");
+    });
+}
+
+int bn_first_hit_noalloc(bn_ctx* ctx, int32_t scene, const bn_pt_params* p, int32_t sample_id, int32_t* inst,
+                         int32_t* prim, float* t) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!p || !inst || !prim || !t) throw Error(BN_E_INVALID, "bad arguments");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        first_hit_noalloc(c, s, *p, sample_id, inst, prim, t);
     });
 }
 

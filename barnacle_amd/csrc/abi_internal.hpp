@@ -116,4 +116,15 @@ struct Ctx {
     }
 };
 
+// Viewport constants of the scene's camera for a film size (abi.hip).
+void camera_constants(SceneRec& sc, int W, int H);
+
+// Allocation-free first hit (noalloc.hip): a static device array holds its
+// scratch; noalloc_init records the array's address on a device (called by
+// bn_create with that device current).
+constexpr size_t kNoAllocMaxPix = (size_t)1 << 20;
+void noalloc_init(int device);
+void first_hit_noalloc(Ctx* c, SceneRec& s, const bn_pt_params& p, int sample_id, int32_t* inst, int32_t* prim,
+                       float* t);
+
 }  // namespace bnx

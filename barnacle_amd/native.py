@@ -50,6 +50,8 @@ _SIGS = {
     "bn_render_pt": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p, C.POINTER(abi.TraceStats)], C.c_int),
     "bn_render_normal": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p], C.c_int),
     "bn_first_hit": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p], C.c_int),
+    "bn_first_hit_noalloc": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p],
+                             C.c_int),
     "bn_set_option": ([C.c_void_p, C.c_char_p, C.c_int64], C.c_int),
     "bn_kernel_times": ([C.c_void_p, C.c_void_p, i32p], C.c_int),
     "bn_reset_kernel_times": ([C.c_void_p], C.c_int),
@@ -197,6 +199,14 @@ class DeviceScene:
         inst, prim, t = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32)
         check(self.ctx.lib.bn_first_hit(self.ctx.h, self.id, C.byref(params), sample_id, _p(inst, i32p),
                                         _p(prim, i32p), _p(t, f32p)))
+        return inst, prim, t
+
+    def first_hit_noalloc(self, params: abi.PTParams, sample_id: int = 0):
+        """first_hit through the allocation-free entry point (static scratch)."""
+        n = params.width * params.height
+        inst, prim, t = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32)
+        check(self.ctx.lib.bn_first_hit_noalloc(self.ctx.h, self.id, C.byref(params), sample_id, _p(inst, i32p),
+                                                _p(prim, i32p), _p(t, f32p)))
         return inst, prim, t
 
     def render_pssmlt(self, params: abi.MLTParams):

@@ -268,6 +268,18 @@ int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
 int bn_first_hit(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
                  int32_t sample_id, int32_t* inst, int32_t* prim, float* t);
 
+/* bn_first_hit without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and outputs as bn_first_hit, for films of at most 2^20
+ * pixels (BN_E_CAPACITY beyond).  Its scratch -- fetch counter, result
+ * staging, traversal-stack spill, 40 MiB in all -- is one static device
+ * array per GPU, set up by bn_create and held until the process ends; every
+ * call on a GPU shares it, so calls that overlap on one GPU (from several
+ * contexts or threads) race.  The scene must already be resident. */
+int bn_first_hit_noalloc(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
+                         int32_t sample_id, int32_t* inst, int32_t* prim, float* t);
+
 /* Batched rendering (no reference counterpart: the reference renders one
  * scene per Scene.Render).  Each item is one PathTracingIntegrator frame of
  * a scene from bn_scene_build / bn_upload_scene; all items advance through
