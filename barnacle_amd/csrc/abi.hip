@@ -152,7 +152,7 @@ void validate_bvh(const std::vector<bn_bvh_node>& nodes, int n_items, const char
 // Concatenate the referenced BLAS and upload the whole scene.
 // Device material records: (kind bits, albedo) (ior, alpha, metallic,
 // roughness).  PBRMaterial: Alpha = Max(roughness^2, 1e-3), Metallic =
-// Clamp(m, 0, 1) (PBR.fs:86-87); the raw roughness is kept for d alpha.
+// Clamp(m, 0, 1) (PBR.fs:11-12); the raw roughness is kept for d alpha.
 std::vector<float4> pack_materials(const std::vector<bn_material>& mats) {
     std::vector<float4> d(2 * std::max<size_t>(1, mats.size()));
     for (size_t i = 0; i < mats.size(); ++i) {

@@ -55,14 +55,14 @@ __device__ __forceinline__ Mat load_mat(const DevScene& S, int m) {
 }
 struct BSDF { V3 f; float pdf; };
 
-__device__ __forceinline__ float pbr_lambda(float alpha, V3 w) {         // PBR.fs:88-94
+__device__ __forceinline__ float pbr_lambda(float alpha, V3 w) {         // PBR.fs:13-19
     float s2 = fmaf(w.x, w.x, w.y * w.y);
     if (s2 == 0.0f) return 0.0f;
     float t2 = s2 / (w.z * w.z);
     float a2t2 = alpha * alpha * t2;
     return (-1.0f + sqrtf(1.0f + a2t2)) / 2.0f;
 }
-__device__ __forceinline__ BSDF pbr_eval(const Mat& m, V3 wo, V3 wi) {   // PBR.fs:112-124
+__device__ __forceinline__ BSDF pbr_eval(const Mat& m, V3 wo, V3 wi) {   // PBR.fs:37-49
     V3 wh = normalize(wo + wi);
     float c = fabsf(wh.z);
     float x = 1.0f + fmaf(m.alpha, m.alpha, -1.0f) * c * c;
@@ -125,7 +125,7 @@ __device__ __forceinline__ BSDF mat_sample(const Mat& m, V3 wo, float ul, V2 u, 
         wi = {-wo.x * ior, -wo.y * ior, -copysignf(ct, wo.z)};
         return {(1.0f - r) * m.albedo, 1.0f - r};
     }
-    // PBR.fs:125-139
+    // PBR.fs:50-64
     if (ul < 1.0f - 0.5f * (1.0f - m.metallic)) {
         float th = atan_f(m.alpha * sqrtf(u.x / (1.0f - u.x)));
         float ph = 2.0f * kPi * u.y;
@@ -218,7 +218,7 @@ __device__ __forceinline__ BSDFGrad mat_sample_grad(const Mat& m, V3 wo, float u
     r.dpdf = 0.0f;
     return r;
 }
-// d alpha / d roughness of Alpha = Max(roughness^2, 1e-3) (PBR.fs:86)
+// d alpha / d roughness of Alpha = Max(roughness^2, 1e-3) (PBR.fs:11)
 __device__ __forceinline__ float dalpha_drough(const Mat& m) {
     float r2 = m.roughness * m.roughness;
     return r2 > 1e-3f ? 2.0f * m.roughness : 0.0f;

@@ -618,19 +618,19 @@ bool anyHit(const orc_scene& s, const Ray& r, float t, Counters& c) {  // BVH.fs
 struct BSDFEval { V3 bsdf; float pdf; };
 struct BSDFSample { BSDFEval e; V3 wi; };
 
-float pbrLambda(const Material& m, V3 w) {                // PBR.fs:88-94
+float pbrLambda(const Material& m, V3 w) {                // PBR.fs:13-19
     float s2 = std::fma(w.x, w.x, w.y * w.y);
     if (s2 == 0.0f) return 0.0f;
     float t2 = s2 / (w.z * w.z);
     float a2t2 = m.alpha * m.alpha * t2;
     return (-1.0f + std::sqrt(1.0f + a2t2)) / 2.0f;
 }
-float pbrD(const Material& m, V3 wh) {                    // PBR.fs:95-98
+float pbrD(const Material& m, V3 wh) {                    // PBR.fs:20-23
     float c = std::fabs(wh.z);
     float x = 1.0f + std::fma(m.alpha, m.alpha, -1.0f) * c * c;
     return m.alpha * m.alpha / (PI_F * (x * x));
 }
-BSDFEval pbrEval(const Material& m, V3 wo, V3 wi) {       // PBR.fs:112-124
+BSDFEval pbrEval(const Material& m, V3 wo, V3 wi) {       // PBR.fs:37-49
     V3 wh = normalize(wo + wi);
     float d = pbrD(m, wh);
     float g = 1.0f / (1.0f + pbrLambda(m, wo) + pbrLambda(m, wi));
@@ -688,7 +688,7 @@ BSDFSample matSample(const Material& m, V3 wo, float uLobe, V2 u) {
             float cosT = std::sqrt(c2t);
             return {{(1.0f - r) * m.albedo, 1.0f - r}, {-wo.x * ior, -wo.y * ior, -std::copysign(cosT, wo.z)}};
         }
-        default: {                                          // PBR.fs:125-139
+        default: {                                          // PBR.fs:50-64
             V3 wi;
             if (uLobe < 1.0f - 0.5f * (1.0f - m.metallic)) {
                 float th = atanF(m.alpha * std::sqrt(u.x / (1.0f - u.x)));
@@ -712,9 +712,9 @@ Material toMaterial(const orc_material& d) {
     m.kind = d.kind;
     m.albedo = {d.albedo[0], d.albedo[1], d.albedo[2]};
     m.ior = d.ior;
-    m.alpha = maxN(d.roughness * d.roughness, 1e-3f);      // PBR.fs:86
+    m.alpha = maxN(d.roughness * d.roughness, 1e-3f);      // PBR.fs:11
     m.roughness = d.roughness;
-    float mt = d.metallic;                                  // Single.Clamp (PBR.fs:87)
+    float mt = d.metallic;                                  // Single.Clamp (PBR.fs:12)
     m.metallic = mt < 0.0f ? 0.0f : (mt > 1.0f ? 1.0f : mt);
     return m;
 }
@@ -1202,6 +1202,15 @@ int orc_material_eval(const orc_material* md, const float* wo, const float* wi, 
     return 0;
 }
 
+int orc_material_eval_n(const orc_material* md, int32_t n, const float* wo, const float* wi, float* out4) {
+    Material m = toMaterial(*md);
+    for (int32_t k = 0; k < n; ++k) {
+        BSDFEval e = matEval(m, {wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]}, {wi[3 * k], wi[3 * k + 1], wi[3 * k + 2]});
+        out4[4 * k] = e.bsdf.x; out4[4 * k + 1] = e.bsdf.y; out4[4 * k + 2] = e.bsdf.z; out4[4 * k + 3] = e.pdf;
+    }
+    return 0;
+}
+
 int orc_scene_create(const orc_scene_desc* d, orc_scene** out) {
     auto* s = new orc_scene();
     int rc;
@@ -1261,7 +1270,7 @@ static void fillStats(const Stats& a, orc_stats* o) {
 // outcomes and the importance factors 1/pdf are held fixed; the MIS weights
 // are differentiated (they depend on roughness through the BSDF pdf).
 
-// PBRMaterial.Eval value (PBR.fs:112-124) in double.
+// PBRMaterial.Eval value (PBR.fs:37-49) in double.
 void pbrFD(const double alb[3], double alpha, double met, const double wo[3], const double wi[3], double f[3],
            double* pdf = nullptr) {
     double h[3] = {wo[0] + wi[0], wo[1] + wi[1], wo[2] + wi[2]};
@@ -1570,6 +1579,15 @@ int orc_camera_ray(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, i
     V2 uPix = smp.next2();
     V2 uLens = smp.next2();
     Ray r = generatePrimary(*s, p->width, p->height, x, y, uPix, uLens);
+    out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
+    out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+    return 0;
+}
+
+// Camera ray of pixel (x, y) for given uPixel / uLens draws (Camera.fs:12-20).
+int orc_camera_ray_u(orc_scene* s, int32_t W, int32_t H, int32_t x, int32_t y, const float* uPix, const float* uLens,
+                     float* out) {
+    Ray r = generatePrimary(*s, W, H, x, y, {uPix[0], uPix[1]}, {uLens[0], uLens[1]});
     out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
     out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
     return 0;

@@ -88,6 +88,7 @@ int orc_bvh_build(const float* bounds, int32_t n, orc_bvh_node* nodes, int32_t* 
  * out = {bsdf.x, bsdf.y, bsdf.z, pdf, wi.x, wi.y, wi.z} */
 int orc_material_sample(const orc_material* m, const float* wo, float u_lobe, const float* u2, float* out7);
 int orc_material_eval(const orc_material* m, const float* wo, const float* wi, float* out4);
+int orc_material_eval_n(const orc_material* m, int32_t n, const float* wo, const float* wi, float* out4);
 
 int  orc_scene_create(const orc_scene_desc* d, orc_scene** out);
 void orc_scene_destroy(orc_scene* s);
@@ -118,7 +119,7 @@ int orc_set_params(orc_scene* s, const float* mat_params, int32_t n_mat, const f
  * MIS weights held fixed (the definition the HIP adjoint implements). */
 int orc_adjoint_pt(orc_scene* s, const orc_pt_params* p, const float* d_film, int32_t n_threads, int32_t x0,
                    int32_t y0, int32_t x1, int32_t y1, double* d_mat, double* d_light);
-/* PBRMaterial.Eval (PBR.fs:112-124) in double and its analytic derivatives,
+/* PBRMaterial.Eval (PBR.fs:37-49) in double and its analytic derivatives,
  * for checking them numerically: f[0..2] = f, f[3] = pdf; d_albedo[c] =
  * d f_c / d albedo_c; d_alpha[0..2] = d f / d alpha, d_alpha[3] = d pdf / d alpha. */
 int orc_pbr_grad_d(const double* albedo, double alpha, double metallic, const double* wo, const double* wi,
@@ -127,6 +128,8 @@ int orc_pbr_grad_d(const double* albedo, double alpha, double metallic, const do
 /* Camera ray of sample sample_id of pixel (x,y) (Integrator.fs:35-39,
  * Camera.fs:12-20): origin in out[0..2], direction in out[3..5]. */
 int orc_camera_ray(orc_scene* s, const orc_pt_params* p, int32_t x, int32_t y, int32_t sample_id, float* out6);
+int orc_camera_ray_u(orc_scene* s, int32_t W, int32_t H, int32_t x, int32_t y, const float* u_pix,
+                     const float* u_lens, float* out6);
 
 /* First hit of sample sample_id of every pixel (pixel order y*W+x). */
 int orc_first_hit(orc_scene* s, const orc_pt_params* p, int32_t sample_id,

@@ -77,6 +77,8 @@ class Oracle:
         L.orc_bvh_build.argtypes = [f32p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p]
         L.orc_material_sample.argtypes = [C.POINTER(abi.Material), f32p, C.c_float, f32p, f32p]
         L.orc_material_eval.argtypes = [C.POINTER(abi.Material), f32p, f32p, f32p]
+        L.orc_material_eval_n.argtypes = [C.POINTER(abi.Material), C.c_int32, f32p, f32p, f32p]
+        L.orc_camera_ray_u.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f32p, f32p, f32p]
 
     def _check(self, rc):
         if rc != 0:
@@ -145,6 +147,14 @@ class Oracle:
         self.lib.orc_material_eval(C.byref(mat), _ptr(wo, f32p), _ptr(wi, f32p), _ptr(out, f32p))
         return out
 
+    def material_eval_n(self, mat: abi.Material, wo, wi):
+        """Eval over arrays of directions (n, 3): (n, 4) = bsdf rgb, pdf."""
+        wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+        wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+        out = np.zeros((len(wo), 4), np.float32)
+        self.lib.orc_material_eval_n(C.byref(mat), len(wo), _ptr(wo, f32p), _ptr(wi, f32p), _ptr(out, f32p))
+        return out
+
     def scene(self, sc) -> "OracleScene":
         return OracleScene(self, sc)
 
@@ -208,6 +218,15 @@ class OracleScene:
     def camera_ray(self, params, x, y, sample_id=0):
         out = np.zeros(6, np.float32)
         self.o._check(self.o.lib.orc_camera_ray(self.h, C.byref(params), x, y, sample_id, _ptr(out, f32p)))
+        return out[:3], out[3:]
+
+    def camera_ray_u(self, width, height, x, y, u_pix, u_lens):
+        """Camera ray of pixel (x, y) for explicit uPixel / uLens draws."""
+        out = np.zeros(6, np.float32)
+        up = np.ascontiguousarray(u_pix, np.float32)
+        ul = np.ascontiguousarray(u_lens, np.float32)
+        self.o._check(self.o.lib.orc_camera_ray_u(self.h, width, height, x, y, _ptr(up, f32p), _ptr(ul, f32p),
+                                                  _ptr(out, f32p)))
         return out[:3], out[3:]
 
     def first_hit(self, params, sample_id=0):

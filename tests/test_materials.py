@@ -128,7 +128,7 @@ def test_ggx_specular_pdf_integrates_to_one(orc):
 
 
 def test_pbr_diffuse_lobe_pdf_mix(orc):
-    """metallic = 0: pdf = mix(spec pdf, cos/pi, 1/2) (PBR.fs:123)."""
+    """metallic = 0: pdf = mix(spec pdf, cos/pi, 1/2) (PBR.fs:48)."""
     m = mat(3, metallic=0.0, roughness=1.0)
     wo = np.array([0, 0, 1], np.float32)
     wi = unit([0.3, 0.2, 0.9])
