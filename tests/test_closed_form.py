@@ -173,3 +173,91 @@ def test_node_transforms_compose_like_the_reference(orc):
             assert inst[k] >= 0 and abs(t[k] - root) <= 1e-4 * root
             n_hits += 1
     assert n_hits > 0.3 * p.width * p.height
+
+
+def two_light_scene(a=30.0, b=20.0, h=100.0, le=(5.0, 2.0), rho=0.5, spp=1, depth=2, integrator="path-tracing"):
+    """The floor of direct_light_scene under two lights at height h: one
+    centred ([-a, a] x [-b, b]), one beside it ([a, 3a] x [-b, b]), with
+    different radiances, so that a wrong light-selection probability or a
+    light picked by the wrong share of uSelect moves the result."""
+    S = 1000.0
+    floor = np.array([[-S, 0, -S], [S, 0, -S], [S, 0, S], [-S, 0, S]], np.float32)
+    l1 = np.array([[-a, h, -b], [a, h, -b], [a, h, b], [-a, h, b]], np.float32)
+    l2 = np.array([[a, h, -b], [3 * a, h, -b], [3 * a, h, b], [a, h, b]], np.float32)
+    quad = np.array([0, 1, 2, 0, 2, 3], np.int32)
+    return {
+        "root": 0,
+        "nodes": [{"instances": [0, 1, 2], "children": [1]}, {"transform": 0, "has-camera": True}],
+        "instances": [{"primitive": 0, "material": 0}, {"primitive": 1, "light": 0}, {"primitive": 2, "light": 1}],
+        "transforms": [{"keyframes": [{"rotation": [-1.5707963, 0.0, 0.0], "translation": [0.0, 50.0, 0.0]}]}],
+        "primitives": [{"type": "mesh", "vertices": floor.reshape(-1), "indices": quad},
+                       {"type": "mesh", "vertices": l1.reshape(-1), "indices": quad},
+                       {"type": "mesh", "vertices": l2.reshape(-1), "indices": quad}],
+        "materials": [{"type": "lambertian", "albedo": [rho, rho, rho]}],
+        "lights": [{"type": "diffuse", "emission": [le[0]] * 3}, {"type": "diffuse", "emission": [le[1]] * 3}],
+        "integrator": {"type": integrator, "spp": spp, "max-depth": depth},
+        "camera": {"type": "pinhole", "fov": 0.2},
+        "film": {"width": 1, "height": 1, "tone-mapping": "identity"},
+    }
+
+
+def two_light_form_factors(a=30.0, b=20.0, h=100.0):
+    f1 = 4 * corner_form_factor(a / h, b / h)
+    f2 = 2 * (corner_form_factor(3 * a / h, b / h) - corner_form_factor(a / h, b / h))
+    return f1, f2
+
+
+def test_two_lights_uniform_selection(orc):
+    """UniformLightSampler (Uniform.fs:13-29) with two lights: the direct
+    radiance is rho (Le1 F1 + Le2 F2), F from the corner form factors by
+    inclusion-exclusion.  Picking light i with a probability other than 1/N,
+    or the triangle with the unscaled uSelect, changes it."""
+    le, rho = (5.0, 2.0), 0.5
+    sc = scene.load_scene(two_light_scene(le=le, rho=rho))
+    osc = orc.scene(sc)
+    p = sc.pt_params()
+    n = 20000
+    vals = np.array([osc.trace_sample(p, 0, 0, k)[0] for k in range(n)], np.float64)
+    f1, f2 = two_light_form_factors()
+    ref = rho * (le[0] * f1 + le[1] * f2)
+    mean, se = vals.mean(), vals.std() / np.sqrt(n)
+    assert abs(mean - ref) < 4 * se + 1e-4 * ref, (mean, ref, se)
+    assert se / ref < 0.01
+
+
+def direct_bsdf_term(a, b, h, rho, le, n_lights, nq=600):
+    """Expectation of DirectIntegrator's BSDF-sampled term (Direct.fs:65-72):
+    f L / pdf_light at a cosine-sampled direction, with pdf_light the light
+    sampler's solid-angle density d^2 / (A cos_l N).  For a parallel
+    rectangle at height h, cos = cos_l = h / r, and the expectation is
+    rho Le N A / pi^2 * integral of h^4 / r^8 over the rectangle (float64
+    midpoint quadrature)."""
+    x = ((np.arange(nq) + 0.5) / nq) * (a[1] - a[0]) + a[0]
+    z = ((np.arange(nq) + 0.5) / nq) * (b[1] - b[0]) + b[0]
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    r2 = X * X + Z * Z + h * h
+    area = (a[1] - a[0]) * (b[1] - b[0])
+    integral = np.sum(h ** 4 / r2 ** 4) * area / nq ** 2
+    return rho * le * n_lights * area / np.pi ** 2 * integral
+
+
+def test_direct_integrator_closed_form(orc):
+    """DirectIntegrator (Direct.fs:10-40) adds, unweighted, the light-sampled
+    direct term (rho Le F) and a BSDF-sampled term divided by the light pdf
+    rather than the BSDF pdf, whose expectation is direct_bsdf_term.  Both
+    light rectangles contribute to both terms."""
+    A, B, h, rho, le = 30.0, 20.0, 100.0, 0.5, (5.0, 2.0)
+    sc = scene.load_scene(two_light_scene(A, B, h, le, rho, spp=2000, integrator="direct"))
+    osc = orc.scene(sc)
+    frames = []
+    for f in range(24):
+        p = sc.pt_params(frame_id=f)
+        frames.append(osc.render(p, integrator=1, threads=8)[0][0, 0])
+    frames = np.array(frames, np.float64)
+    f1, f2 = two_light_form_factors(A, B, h)
+    ref = rho * (le[0] * f1 + le[1] * f2)
+    ref += direct_bsdf_term((-A, A), (-B, B), h, rho, le[0], 2)
+    ref += direct_bsdf_term((A, 3 * A), (-B, B), h, rho, le[1], 2)
+    mean, se = frames.mean(), frames.std() / np.sqrt(len(frames))
+    assert abs(mean - ref) < 4 * se + 2e-4 * ref, (mean, ref, se)
+    assert se / ref < 0.01
