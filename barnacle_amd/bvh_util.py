@@ -10,3 +10,10 @@ NODE_DTYPE = np.dtype([("pmin", np.float32, 3), ("pmax", np.float32, 3), ("right
 def nodes_to_numpy(nodes, n):
     raw = C.string_at(C.addressof(nodes), NODE_DTYPE.itemsize * n)
     return np.frombuffer(raw, NODE_DTYPE).copy()
+
+
+def numpy_to_nodes(arr):
+    """The inverse of nodes_to_numpy: a ctypes BVHNode array of the same bytes."""
+    from .abi import BVHNode
+    a = np.ascontiguousarray(arr, NODE_DTYPE)
+    return (BVHNode * max(1, len(a))).from_buffer_copy(a.tobytes() if len(a) else bytes(32))

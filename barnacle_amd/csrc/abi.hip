@@ -149,6 +149,14 @@ void validate_bvh(const std::vector<bn_bvh_node>& nodes, int n_items, const char
     }
 }
 
+// A released BLAS keeps its id (ids are never reused) and frees its host copy.
+void release_blas(BlasRec& b) {
+    b.live = false;
+    std::vector<bn_bvh_node>().swap(b.nodes);
+    std::vector<float>().swap(b.tris);
+    b.alias = bnh::AliasTable{};
+}
+
 // Concatenate the referenced BLAS and upload the whole scene.
 // Device material records: (kind bits, albedo) (ior, alpha, metallic,
 // roughness).  PBRMaterial: Alpha = Max(roughness^2, 1e-3), Metallic =
@@ -180,24 +188,24 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
                   const std::vector<bn_instance_upload>& insts, const std::vector<bn_material>& mats,
                   const std::vector<bn_light>& lights, const bn_camera& cam, const float* c2w) {
     for (const auto& in : insts) {
-        if (in.blas >= (int)c->blas.size() || (in.blas < 0 && !(in.sphere_radius == in.sphere_radius)))
+        if (in.blas >= (int)c->blas.size() || (in.blas >= 0 && !c->blas[in.blas].live) ||
+            (in.blas < 0 && !(in.sphere_radius == in.sphere_radius)))
             throw Error(BN_E_INVALID, "instance references an unknown BLAS");
-        if (in.material >= (int)mats.size() || in.light >= (int)lights.size())
+        if (in.material >= (int)mats.size() || in.light >= (int)lights.size() || in.material < -1 || in.light < -1)
             throw Error(BN_E_INVALID, "instance references an unknown material/light");
     }
     if (tlas.empty()) throw Error(BN_E_INVALID, "empty TLAS");
     validate_bvh(tlas, (int)insts.size(), "TLAS");
     // BLAS pool: nodes keep their BLAS-local child indices; each instance
     // records where its BLAS starts in the node and triangle pools.
-    std::vector<int> node_base(c->blas.size(), -1), tri_base(c->blas.size(), -1);
+    std::map<int, std::pair<int, int>> base;      // BLAS id -> (first node, first triangle)
     std::vector<bn_bvh_node> nodes;
     std::vector<float> tris, aprob, apdf;
     std::vector<int> aidx;
     for (const auto& in : insts) {
-        if (in.blas < 0 || node_base[in.blas] >= 0) continue;
+        if (in.blas < 0 || base.count(in.blas)) continue;
         const BlasRec& b = c->blas[in.blas];
-        node_base[in.blas] = (int)nodes.size();
-        tri_base[in.blas] = (int)(tris.size() / 9);
+        base[in.blas] = {(int)nodes.size(), (int)(tris.size() / 9)};
         nodes.insert(nodes.end(), b.nodes.begin(), b.nodes.end());
         tris.insert(tris.end(), b.tris.begin(), b.tris.end());
         aprob.insert(aprob.end(), b.alias.prob.begin(), b.alias.prob.end());
@@ -229,7 +237,7 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
         }
         d.lo = make_float4(in.bounds[0], in.bounds[1], in.bounds[2], in.sphere_radius);
         d.hi = make_float4(in.bounds[3], in.bounds[4], in.bounds[5], pdf_const);
-        d.info = make_int4(in.blas >= 0 ? node_base[in.blas] : -1, in.blas >= 0 ? tri_base[in.blas] : 0,
+        d.info = make_int4(in.blas >= 0 ? base[in.blas].first : -1, in.blas >= 0 ? base[in.blas].second : 0,
                            in.material, in.light);
         d.aux = make_int4(is_identity_bits(in.w2o) ? 1 : 0, 0, 0, 0);
         if (in.light >= 0) lrec.push_back(make_int4((int)i, d.info.y, ntri, in.light));
@@ -665,7 +673,8 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         paths_total += (size_t)base[k].p_pad * (size_t)std::max(p.spp, 0);
     }
     const size_t cap = std::max(max_pad, std::min(window_paths(c), paths_total));
-    if (cap > ((size_t)1 << 31)) throw Error(BN_E_CAPACITY, "round larger than 2^31 paths");
+    // path ids and RoundItem path counts are int32: rounds stay below 2^31 paths
+    if (cap > ((size_t)1 << 30)) throw Error(BN_E_CAPACITY, "round larger than 2^30 paths");
     const size_t slots = std::min(cap, c->chunk_slots);
     ensure_paths(c, slots);
     ensure_spill(c);
@@ -1096,7 +1105,7 @@ int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes, const
                    const int32_t* tri_idx, int32_t n_tris, const float* alias_pdf, int32_t* out_blas) {
     return guard([&] {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
-        if (!c || !nodes || n_nodes <= 0 || !verts || !tri_idx || n_tris <= 0 || !out_blas)
+        if (!c || !nodes || n_nodes <= 0 || !verts || n_verts <= 0 || !tri_idx || n_tris <= 0 || !out_blas)
             throw Error(BN_E_INVALID, "bad arguments");
         BlasRec b;
         b.nodes.assign(nodes, nodes + n_nodes);
@@ -1109,9 +1118,15 @@ int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes, const
             }
         std::vector<float> w(n_tris);
         if (alias_pdf) {
-            // The F# host passes the alias table pdfs; the table itself is the
-            // degenerate construction over them (AliasTable.fs:14-51).
-            b.alias = bnh::build_alias(alias_pdf, n_tris);
+            // The F# host passes AliasTable.Table[i].pdf (Mesh.fs:181-186),
+            // kept verbatim: EvalPDF reads Table[0].pdf (Mesh.fs:300-304) and
+            // Sample returns Table[i].pdf.  The reference's table over
+            // non-negative weights is degenerate (AliasTable.fs:20,45-50: every
+            // prob <= 1, so every entry ends with prob 1 and alias = itself).
+            b.alias.pdf.assign(alias_pdf, alias_pdf + n_tris);
+            b.alias.prob.assign(n_tris, 1.0f);
+            b.alias.alias.resize(n_tris);
+            for (int i = 0; i < n_tris; ++i) b.alias.alias[i] = i;
         } else {
             for (int i = 0; i < n_tris; ++i) {
                 const float* t = &b.tris[9 * (size_t)i];
@@ -1125,19 +1140,32 @@ int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes, const
     });
 }
 
+int bn_release_blas(bn_ctx* ctx, int32_t blas) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c) throw Error(BN_E_INVALID, "null context");
+        if (blas < 0 || blas >= (int)c->blas.size() || !c->blas[blas].live || c->blas[blas].owner >= 0)
+            throw Error(BN_E_INVALID, "not a live BLAS from bn_upload_blas");
+        release_blas(c->blas[blas]);
+    });
+}
+
 int bn_upload_scene(bn_ctx* ctx, const bn_bvh_node* tlas, int32_t n_tlas, const bn_instance_upload* instances,
                     int32_t n_instances, const bn_material* materials, int32_t n_materials, const bn_light* lights,
                     int32_t n_lights, const bn_camera* camera, const float* c2w, int32_t* out_scene) {
     return guard([&] {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
-        if (!c || !tlas || n_tlas <= 0 || !instances || n_instances <= 0 || !camera || !c2w || !out_scene)
+        if (!c || !tlas || n_tlas <= 0 || !instances || n_instances <= 0 || !camera || !c2w || !out_scene ||
+            n_materials < 0 || n_lights < 0 || (n_materials > 0 && !materials) || (n_lights > 0 && !lights))
             throw Error(BN_E_INVALID, "bad arguments");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto sc = std::make_unique<SceneRec>();
         upload_scene(c, *sc, std::vector<bn_bvh_node>(tlas, tlas + n_tlas),
                      std::vector<bn_instance_upload>(instances, instances + n_instances),
-                     std::vector<bn_material>(materials, materials + n_materials),
-                     std::vector<bn_light>(lights, lights + n_lights), *camera, c2w);
+                     n_materials ? std::vector<bn_material>(materials, materials + n_materials)
+                                 : std::vector<bn_material>(),
+                     n_lights ? std::vector<bn_light>(lights, lights + n_lights) : std::vector<bn_light>(), *camera,
+                     c2w);
         c->scenes.push_back(std::move(sc));
         *out_scene = (int)c->scenes.size() - 1;
     });
@@ -1166,6 +1194,7 @@ int bn_scene_build(bn_ctx* ctx, const bn_scene_desc* desc, int32_t* out_scene) {
                     b.tris[9 * (size_t)k + 3 * j + 2] = v.z;
                 }
             b.alias = m.alias;
+            b.owner = (int)c->scenes.size();      // the id this scene gets below
             validate_bvh(b.nodes, nt, "BLAS");
             c->blas.push_back(std::move(b));
             blas_of[i] = (int)c->blas.size() - 1;
@@ -1184,7 +1213,13 @@ int bn_scene_build(bn_ctx* ctx, const bn_scene_desc* desc, int32_t* out_scene) {
             u.light = h.light;
         }
         auto sc = std::make_unique<SceneRec>();
-        upload_scene(c, *sc, hs->tlas, ups, hs->materials, hs->light_defs, hs->camera, hs->c2w.m);
+        try {
+            upload_scene(c, *sc, hs->tlas, ups, hs->materials, hs->light_defs, hs->camera, hs->c2w.m);
+        } catch (...) {
+            for (int b : blas_of)
+                if (b >= 0) release_blas(c->blas[b]);
+            throw;
+        }
         sc->host = hs;
         c->scenes.push_back(std::move(sc));
         *out_scene = (int)c->scenes.size() - 1;
@@ -1198,6 +1233,9 @@ int bn_scene_release(bn_ctx* ctx, int32_t scene) {
         hip_check(hipStreamSynchronize(c->stream), "sync");
         if (c->fwd.scene == &s) c->fwd.valid = false;
         c->scenes[scene] = std::make_unique<SceneRec>();
+        // the BLAS this scene's build made; its device copies went with the scene
+        for (BlasRec& b : c->blas)
+            if (b.live && b.owner == scene) release_blas(b);
     });
 }
 
@@ -1242,6 +1280,31 @@ int bn_scene_export_blas(bn_ctx* ctx, int32_t scene, int32_t prim, bn_bvh_node* 
         std::memcpy(tri, m.tri.data(), m.tri.size() * 4);
         std::memcpy(pdf, m.alias.pdf.data(), nt * 4);
         *n_tris = nt;
+    });
+}
+
+int bn_scene_export_instances(bn_ctx* ctx, int32_t scene, bn_instance_upload* out, int32_t* n) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        SceneRec& s = get_scene(c, scene);
+        if (!s.host) throw Error(BN_E_INVALID, "scene was not built by bn_scene_build");
+        if (!out || !n) throw Error(BN_E_INVALID, "bad arguments");
+        const bnh::HostScene& h = *s.host;
+        if ((int)h.insts.size() > *n) throw Error(BN_E_CAPACITY, "export buffer too small");
+        for (size_t i = 0; i < h.insts.size(); ++i) {
+            const bnh::HostInstance& hi = h.insts[i];
+            bn_instance_upload& u = out[i];
+            std::memcpy(u.o2w, hi.o2w.m, 64);
+            std::memcpy(u.w2o, hi.w2o.m, 64);
+            u.bounds[0] = hi.bounds.lo.x; u.bounds[1] = hi.bounds.lo.y; u.bounds[2] = hi.bounds.lo.z;
+            u.bounds[3] = hi.bounds.hi.x; u.bounds[4] = hi.bounds.hi.y; u.bounds[5] = hi.bounds.hi.z;
+            const bool sphere = h.meshes[hi.prim].sphere;
+            u.blas = sphere ? -1 : hi.prim;
+            u.sphere_radius = sphere ? h.meshes[hi.prim].radius : 0.0f;
+            u.material = hi.material;
+            u.light = hi.light;
+        }
+        *n = (int)h.insts.size();
     });
 }
 
@@ -1324,8 +1387,8 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "graph") {
             c->use_graph = value != 0;
         } else if (k == "window_paths") {
-            if (value != 0 && (value < 64 || value > ((int64_t)1 << 31)))
-                throw Error(BN_E_INVALID, "window_paths out of range");
+            if (value != 0 && (value < 64 || value > ((int64_t)1 << 30)))
+                throw Error(BN_E_INVALID, "window_paths out of range [64, 2^30]");
             c->window_paths = (size_t)value;
         } else if (k == "material_sort") {
             if (value < -1 || value > 1) throw Error(BN_E_INVALID, "material_sort must be -1, 0 or 1");

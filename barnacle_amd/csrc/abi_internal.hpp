@@ -35,6 +35,10 @@ struct BlasRec {
     std::vector<bn_bvh_node> nodes;
     std::vector<float> tris;     // 9 floats per triangle
     bnh::AliasTable alias;
+    bool live = true;            // false once released (its host copy is freed)
+    int owner = -1;              // scene whose bn_scene_build made it (released with
+                                 // that scene); -1: bn_upload_blas, released by
+                                 // bn_release_blas or bn_destroy
 };
 
 struct SceneRec {

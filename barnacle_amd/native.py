@@ -47,6 +47,8 @@ _SIGS = {
     "bn_upload_scene": ([C.c_void_p, C.POINTER(abi.BVHNode), C.c_int32, C.POINTER(abi.InstanceUpload),
                          C.c_int32, C.POINTER(abi.Material), C.c_int32, C.POINTER(abi.Light), C.c_int32,
                          C.POINTER(abi.Camera), f32p, i32p], C.c_int),
+    "bn_release_blas": ([C.c_void_p, C.c_int32], C.c_int),
+    "bn_scene_export_instances": ([C.c_void_p, C.c_int32, C.POINTER(abi.InstanceUpload), i32p], C.c_int),
     "bn_render_pt": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p, C.POINTER(abi.TraceStats)], C.c_int),
     "bn_render_normal": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p], C.c_int),
     "bn_first_hit": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p], C.c_int),
@@ -141,6 +143,40 @@ class Context:
     def scene(self, sc) -> "DeviceScene":
         return DeviceScene(self, sc)
 
+    def upload_blas(self, nodes, vertices, tri_indices, alias_pdf=None) -> int:
+        """bn_upload_blas: a BLAS built by the caller (nodes in the 32-byte
+        Util/BVH.fs layout, vertices (n, 3), triangle vertex indices in BVH
+        order (t, 3), AliasTable pdfs per triangle or None)."""
+        from .bvh_util import numpy_to_nodes
+        nd = numpy_to_nodes(nodes)
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+        t = np.ascontiguousarray(tri_indices, np.int32).reshape(-1)
+        pdf = None if alias_pdf is None else np.ascontiguousarray(alias_pdf, np.float32)
+        out = C.c_int32()
+        check(self.lib.bn_upload_blas(self.h, nd, len(nodes), _p(v, f32p), len(v) // 3, _p(t, i32p), len(t) // 3,
+                                      None if pdf is None else _p(pdf, f32p), C.byref(out)))
+        return out.value
+
+    def release_blas(self, blas: int):
+        check(self.lib.bn_release_blas(self.h, blas))
+
+    def upload_scene(self, sc, tlas, instances, camera_to_world) -> "DeviceScene":
+        """bn_upload_scene: TLAS nodes, bn_instance_upload records (TLAS
+        order), the materials, lights and camera of scene `sc`; returns a
+        DeviceScene over the uploaded id."""
+        from .bvh_util import numpy_to_nodes
+        tl = numpy_to_nodes(tlas)
+        n = len(instances)
+        ins = (abi.InstanceUpload * n)(*instances)
+        desc, keep = sc.desc()
+        c2w = np.ascontiguousarray(camera_to_world, np.float32)
+        out = C.c_int32()
+        check(self.lib.bn_upload_scene(self.h, tl, len(tlas), ins, n, desc.materials, desc.n_materials,
+                                       desc.lights, desc.n_lights, C.byref(desc.camera), _p(c2w, f32p),
+                                       C.byref(out)))
+        del keep
+        return DeviceScene(self, sc, scene_id=out.value)
+
     def render_batch(self, jobs):
         """jobs: list of (DeviceScene, PTParams).  One batched launch sequence;
         returns (films, {"rounds", "graph"})."""
@@ -171,13 +207,25 @@ class Context:
 class DeviceScene:
     """A scene built on the host (BLAS/TLAS) and uploaded to the device."""
 
-    def __init__(self, ctx: Context, sc):
+    def __init__(self, ctx: Context, sc, scene_id: int | None = None):
         self.ctx, self.sc = ctx, sc
-        desc, keep = sc.desc()
-        sid = C.c_int32()
-        check(ctx.lib.bn_scene_build(ctx.h, C.byref(desc), C.byref(sid)))
-        self.id = sid.value
+        if scene_id is None:
+            desc, keep = sc.desc()
+            sid = C.c_int32()
+            check(ctx.lib.bn_scene_build(ctx.h, C.byref(desc), C.byref(sid)))
+            scene_id = sid.value
+        self.id = scene_id
         self.param_gen = 0
+
+    def release(self):
+        check(self.ctx.lib.bn_scene_release(self.ctx.h, self.id))
+
+    def export_instances(self):
+        """bn_instance_upload records of the built scene (TLAS order)."""
+        n = C.c_int32(max(1, len(self.sc.instances)))
+        out = (abi.InstanceUpload * n.value)()
+        check(self.ctx.lib.bn_scene_export_instances(self.ctx.h, self.id, out, C.byref(n)))
+        return [out[i] for i in range(n.value)]
 
     def render_pt(self, params: abi.PTParams, stats: bool = False):
         """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters."""

@@ -238,6 +238,8 @@ typedef struct bn_instance_upload {
     int32_t light;                     /* index into lights,    -1 = none    */
 } bn_instance_upload;
 
+/* alias_pdf: AliasTable.Table[i].pdf per triangle in BVH order, kept
+ * verbatim (NULL: derived from the triangle areas as Mesh.fs:181-186 does). */
 int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes,
                    const float* vertices, int32_t n_verts,
                    const int32_t* tri_indices, int32_t n_tris,
@@ -248,6 +250,17 @@ int bn_upload_scene(bn_ctx* ctx, const bn_bvh_node* tlas, int32_t n_tlas,
                     const bn_light* lights, int32_t n_lights,
                     const bn_camera* camera, const float* camera_to_world,
                     int32_t* out_scene);
+/* Frees the host copy of a BLAS from bn_upload_blas once no further scene
+ * will reference it (scenes keep their own device copies).  BLAS made by
+ * bn_scene_build are released with their scene by bn_scene_release. */
+int bn_release_blas(bn_ctx* ctx, int32_t blas);
+
+/* The instance records bn_scene_build derived (TLAS order), in the form
+ * bn_upload_scene takes them: blas = the primitive index of the scene
+ * description (-1 for an analytic sphere, with its radius), world bounds as
+ * PrimitiveInstance.UpdateTransform computes them (Base/Primitive.fs:131-138).
+ * n: capacity in, count out. */
+int bn_scene_export_instances(bn_ctx* ctx, int32_t scene, bn_instance_upload* out, int32_t* n);
 
 /* PathTracingIntegrator.Render (Extensions/Integrator/PathTracing.fs:14-81
  * through ProgressiveIntegrator.RenderTile/Render, Base/Integrator.fs:22-55):

@@ -147,3 +147,36 @@ def test_adjoint_lambertian_box_and_zero_loss(ctx, orc):
         assert np.allclose(g, o, rtol=1e-2, atol=2e-3 * scale)
     zm, zl = ds.backward_pt(p, np.zeros_like(d))
     assert not zm.any() and not zl.any()
+
+
+def test_adjoint_uncached_and_multi_round_match_cached(ctx, orc):
+    """backward_pt's other two paths: with no forward pass of the same frame
+    cached (after a normal view, which invalidates the cache, the forward
+    rounds run inside backward_pt), and with rounds smaller than the frame
+    (window_paths below the frame's paths: several forward + replay rounds).
+    Both equal the cached single-round gradients up to float atomics order,
+    and the FP64 oracle adjoint."""
+    info, sc = make(max_depth=5, rr_depth=3, spp=8, res=32)
+    p = sc.pt_params()
+    ds = ctx.scene(sc)
+    film = ds.render_pt(p)
+    d = film_adjoint(film)
+    cached = ds.backward_pt(p, d)
+    ds.render_normal(p)                                     # invalidates the cached forward
+    uncached = ds.backward_pt(p, d)
+    om, ol = orc.scene(sc).adjoint_pt(p, d)
+    # 32x32 pixels = 1024 paths per sample: 2048-path rounds give 4 rounds of 2 samples
+    ctx.set_option("window_paths", 2048)
+    try:
+        multi = ds.backward_pt(p, d)
+        film_multi = ds.render_pt(p)
+    finally:
+        ctx.set_option("window_paths", 0)
+    assert np.array_equal(film_multi.view(np.int32), film.view(np.int32))   # rounds do not change the film
+    for g in (uncached, multi):
+        for a, b in zip(g, cached):
+            scale = np.abs(b).max()
+            assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale)
+    for gm, gl in (uncached, multi):
+        for g, o in ((gm[:, :3], om[:, :3]), (gm[:, 3], om[:, 3]), (gl, ol)):
+            assert np.linalg.norm(g - o) <= 5e-3 * np.linalg.norm(o)

@@ -98,3 +98,26 @@ def test_normal_view_spheres(ctx, orc, sph):
     o, _ = orc.scene(sph).render(p, integrator=0)
     assert np.mean(g == o) > 0.999
     assert np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)) <= 1e-5
+
+
+def test_closed_form_scenes_on_the_gpu(ctx, orc):
+    """The closed-form scenes that pin the oracle, rendered on the GPU: inside
+    an emitting Lambertian sphere the film is Le (sum_{k<D} a^k + a^D / 2)
+    (no roulette), and the sphere-light floor point matches the oracle's
+    film bit for bit at equal samples (its expectation is rho Le (r/d)^2)."""
+    import test_oracle_pins as pins
+    a, le, depth = 0.6, 1.5, 8
+    sc = scene.load_scene(pins.sphere_furnace_scene(a, le, depth, rr_depth=100, w=16, h=16, spp=4))
+    g = ctx.scene(sc).render_pt(sc.pt_params())
+    o, _ = orc.scene(sc).render(sc.pt_params(), threads=4)
+    assert np.allclose(g, pins.furnace_radiance(a, le, depth), rtol=2e-5)
+    assert np.array_equal(g, o)
+    sc = scene.load_scene(pins.sphere_light_scene())
+    info = pins.sphere_light_scene()
+    info["integrator"]["spp"] = 4096
+    sc = scene.load_scene(info)
+    p = sc.pt_params()
+    g = ctx.scene(sc).render_pt(p)
+    o, _ = orc.scene(sc).render(p, threads=4)
+    assert np.allclose(g, o, rtol=1e-5)
+    assert g[0, 0] == pytest.approx(0.5 * 5.0 * 0.01, rel=0.05)
