@@ -13,6 +13,7 @@ Asset/cbox.json); they are authored here from the workload descriptions:
                        radially by 4-octave value noise (amplitude 0.08).
   config1()            Cornell box + displaced sphere with one GGX (pbr)
                        material drawn from seed 1; 1024x1024, 64 spp, depth 16.
+  config1_bunny()      config1 with the reference's bunny OBJ as the mesh.
   caustic_scene()      Cornell box + dielectric sphere (IOR 1.5) and a GGX metal
                        sphere (roughness 0.1) of 20,480 triangles each, PSSMLT.
 
@@ -177,6 +178,23 @@ def config1(width=1024, height=1024, spp=64, max_depth=16, freq=59):
     _add_mesh_instance(info, displaced_sphere(freq, 0.08, 1),
                        {"type": "pbr", "albedo": base, "metallic": metallic, "roughness": rough},
                        90.0, (185.5, 300.0, -168.5))
+    return info
+
+
+def config1_bunny(obj_path: str, width=1024, height=1024, spp=64, max_depth=16):
+    """Config 1 with the reference's own mesh asset in place of the displaced
+    sphere: Asset/stanford-bunny.obj (69,451 triangles, read by the OBJ
+    reader of Mesh.fs:246-279 through the scene's "uri"), scaled by 1200 and
+    set on top of the short box, with config 1's GGX material."""
+    import os
+    info = cornell_box(width, height, spp, max_depth)
+    rng = np.random.default_rng(1)
+    base = [float(np.float32(x)) for x in rng.uniform(0.2, 0.9, 3)]
+    metallic = float(rng.integers(0, 2))
+    rough = float(np.float32(rng.uniform(0.05, 0.8)))
+    _add_mesh_instance(info, {"type": "mesh", "uri": os.path.abspath(obj_path)},
+                       {"type": "pbr", "albedo": base, "metallic": metallic, "roughness": rough},
+                       1200.0, (205.9, 125.4, -166.7))
     return info
 
 

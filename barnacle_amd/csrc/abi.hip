@@ -454,15 +454,20 @@ void run_round(Ctx* c, bool batch, bnd::Integrator kind, const bnd::Launch& L, c
     bnd::enqueue_round_end(batch, kind, L, P, ctl, B, hook);
 }
 
-// One frame of PathTracingIntegrator (or the Normal view), in rounds of
-// whole samples of all pixels, round after round in sample order.
-void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, float* film_host, bn_trace_stats* stats) {
+// One frame of PathTracingIntegrator (or the Normal view, or the Direct
+// integrator), in rounds of whole samples of all pixels, round after round in
+// sample order.
+void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bnd::Integrator kind, float* film_host,
+                   bn_trace_stats* stats) {
+    const bool normal = kind == bnd::Integrator::Normal;
     check_pt_params(p);
     if (sc.n_lights == 0) throw Error(BN_E_NO_LIGHT, "LightSamplerBase: No light primitives found.");
     hipStream_t s = c->stream;
     camera_constants(sc, p.width, p.height);
     bnd::RoundItem it = plain_item(p);
     if (normal) it.max_depth = 1;                          // one bounce, whatever max-depth says
+    // DirectIntegrator ignores max-depth: camera hit + one BSDF-sampled hit
+    if (kind == bnd::Integrator::Direct) { it.max_depth = 2; it.rr_depth = 2; }
     const size_t want = std::max<size_t>((size_t)it.p_pad, window_paths(c));
     const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(p.spp, 1), want / it.p_pad));
     const size_t round_paths = (size_t)ns * it.p_pad;
@@ -495,10 +500,10 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bool normal, flo
         it.ns = std::min(ns, p.spp - s0);
         it.path_count = it.ns * it.p_pad;
         L.item = it;
-        run_round(c, false, normal ? bnd::Integrator::Normal : bnd::Integrator::Path, L, P, ctl, B,
-                  (uint32_t)it.path_count, it.max_depth, stats != nullptr, &hook, nullptr);
+        run_round(c, false, kind, L, P, ctl, B, (uint32_t)it.path_count, it.max_depth, stats != nullptr, &hook,
+                  nullptr);
     }
-    if (!normal && ns >= p.spp) {                           // one round: L_out holds the frame
+    if (kind == bnd::Integrator::Path && ns >= p.spp) {     // one round: L_out holds the frame
         c->fwd.valid = true;
         c->fwd.scene = &sc;
         c->fwd.gen = sc.gen;
@@ -854,7 +859,8 @@ void autotune(Ctx* c, const bn_batch_item* items, int n, int flags, bn_tune_resu
             render_batch(c, items, n, nullptr, nullptr);
         } else {
             for (int k = 0; k < n; ++k)
-                render_frames(c, get_scene(c, items[k].scene), items[k].params, false, nullptr, nullptr);
+                render_frames(c, get_scene(c, items[k].scene), items[k].params, bnd::Integrator::Path, nullptr,
+                              nullptr);
         }
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
@@ -1308,13 +1314,23 @@ int bn_scene_export_instances(bn_ctx* ctx, int32_t scene, bn_instance_upload* ou
     });
 }
 
+int bn_film_postprocess(const float* film, int64_t n_pixels, int32_t tone_mapping, float* out_rgb,
+                        uint8_t* out_rgba8) {
+    return guard([&] {
+        if (!film || n_pixels < 0 || (!out_rgb && !out_rgba8)) throw Error(BN_E_INVALID, "bad arguments");
+        if (tone_mapping < BN_TONE_IDENTITY || tone_mapping > BN_TONE_GAMMA)
+            throw Error(BN_E_INVALID, "Unknown tone mapping type");
+        bnh::postprocess(film, n_pixels, tone_mapping, out_rgb, out_rgba8);
+    });
+}
+
 int bn_render_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, float* film, bn_trace_stats* stats) {
     return guard([&] {
         Ctx* c = reinterpret_cast<Ctx*>(ctx);
         if (!params) throw Error(BN_E_INVALID, "null params");
         SceneRec& s = get_scene(c, scene);
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        render_frames(c, s, *params, false, film, stats);
+        render_frames(c, s, *params, bnd::Integrator::Path, film, stats);
     });
 }
 
@@ -1324,7 +1340,17 @@ int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, flo
         if (!params) throw Error(BN_E_INVALID, "null params");
         SceneRec& s = get_scene(c, scene);
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        render_frames(c, s, *params, true, film, nullptr);
+        render_frames(c, s, *params, bnd::Integrator::Normal, film, nullptr);
+    });
+}
+
+int bn_render_direct(bn_ctx* ctx, int32_t scene, const bn_pt_params* params, float* film) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!params) throw Error(BN_E_INVALID, "null params");
+        SceneRec& s = get_scene(c, scene);
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        render_frames(c, s, *params, bnd::Integrator::Direct, film, nullptr);
     });
 }
 

@@ -125,6 +125,11 @@ struct HostScene {
 // Scene.Load + Traverse + BVHAggregate + UniformLightSampler construction.
 void build_scene(const bn_scene_desc& d, HostScene& out);
 
+// Film.PostProcess with tone mapping 0 identity, 1 ACES, 2 gamma, clamped to
+// [0, 1], and the 8-bit RGBA of Film.Save (film.cpp).  Either output may be
+// null.
+void postprocess(const float* film, int64_t n, int tone_mapping, float* out_rgb, uint8_t* out_rgba8);
+
 // Triangle area of Mesh.fs:84-87.
 inline float tri_area(V3 p0, V3 p1, V3 p2) { return 0.5f * length(cross(p1 - p0, p2 - p0)); }
 

@@ -161,7 +161,7 @@ struct LaunchHook {
 // appends new paths after the continuing ones, shading moves the continuing
 // paths to the other copy and retires the ended ones by path id.
 //   iterations: n (even) x [regen, prep, trace_closest, shade, trace_shadow]
-enum class Integrator { Path, Normal, Adjoint };
+enum class Integrator { Path, Normal, Adjoint, Direct };
 void enqueue_round_begin(bool batch, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,
                          LaunchHook* hook);
 void enqueue_iterations(bool batch, Integrator kind, const Launch& L, const PathBuf& P, Ctl* ctl, const Buffers& B,

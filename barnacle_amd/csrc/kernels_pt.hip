@@ -1083,6 +1083,77 @@ __global__ void k_shade_normal(Work<false> W, PathBuf P, Ctl* ctl, const uint32_
     }
 }
 
+// DirectIntegrator.Li (Extensions/Integrator/Direct.fs:10-40) on the bins of
+// a slot round.  Depth 0 (camera hit): emission seen directly, L + Le with no
+// MIS weight; at a material, a light sample whose unweighted term
+// f * Le * (1 / pdf_l) lands through the shadow ray (cast whatever the pdf,
+// as the reference does), then a BSDF sample that continues as the depth-1
+// ray carrying f in beta.  Depth 1: a light hit adds f * Le * (1 / pdf_l),
+// pdf_l the light sampler's Eval at the ray origin; the path ends.  The
+// shadow term of depth 0 lands before the depth-1 term, as in the reference.
+__global__ void k_shade_direct(Work<false> W, PathBuf P, Ctl* ctl, const uint32_t* bins, uint32_t bin_cap,
+                               uint32_t* queue_next, int nxt, uint32_t* shq, uint32_t* freel) {
+    uint32_t start[kBins + 1];
+    start[0] = 0;
+    for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
+    const uint32_t total = start[kBins];
+    const DevScene& S = W.S;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t slot = 0;
+        bool cont = false, shadow = false;
+        if (i < total) {
+            int b = 0;
+            while (i >= start[b + 1]) ++b;
+            slot = bins[(size_t)b * bin_cap + (i - start[b])];
+            const float4 o4 = P.o[slot];
+            const Ray r = {xyz(o4), xyz(P.d[slot])};
+            const int depth = __float_as_int(o4.w);
+            const int4 h = P.hit[slot];
+            const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
+            const int4 info = S.insts[h.x].info;
+            V3 L = xyz(P.L[slot]);
+            if (depth == 0) {
+                if (info.w >= 0) L = L + light_emit(S.lightdefs[info.w], dot(-r.d, g.n));
+                if (info.z >= 0) {
+                    uint32_t rng = P.rng[slot];
+                    const Mat m = load_mat(S, info.z);
+                    const V3 wo_l = to_local(g, -r.d);
+                    const float uSel = lcg(rng);
+                    V2 uL;
+                    uL.x = lcg(rng); uL.y = lcg(rng);
+                    const LightSample ls = light_sample(S, g.p, uSel, uL);
+                    const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
+                    const float dist = length(ls.p - g.p);
+                    P.sh_o[slot] = f4(g.p, dist - 1e-3f);
+                    P.sh_d[slot] = f4(ls.wi, 0.0f);
+                    P.nee_a[slot] = f4(e.f, 0.0f);
+                    P.nee_b[slot] = f4(ls.L * (1.0f / ls.pdf), 0.0f);
+                    shadow = true;
+                    const float ul = lcg(rng);
+                    V2 ub;
+                    ub.x = lcg(rng); ub.y = lcg(rng);
+                    V3 wi_l;
+                    const BSDF bs = mat_sample(m, wo_l, ul, ub, wi_l);
+                    P.o[slot] = f4(g.p, __int_as_float(1));
+                    P.d[slot] = f4(to_world(g, wi_l), 0.0f);
+                    P.beta[slot] = f4(bs.f, 0.0f);
+                    P.rng[slot] = rng;
+                    cont = true;
+                }
+            } else if (info.w >= 0) {
+                V3 Le;
+                float lpdf;
+                light_eval(S, h.x, r.o, g, Le, lpdf);
+                L = fma3(xyz(P.beta[slot]), Le * (1.0f / lpdf), L);
+            }
+            P.L[slot] = f4(L, 0.0f);
+        }
+        block_push<3>({cont, shadow, i < total && !cont}, {&ctl->q_count[nxt], &ctl->sh_count, &ctl->free_count[nxt]},
+                      {queue_next, shq, freel}, slot);
+    }
+}
+
 // First hit of one sample per pixel: (instance, primitive, t) in pixel order.
 __global__ __launch_bounds__(256) void k_first_hit(DevScene S, int width, int height, int sample_id, int spp,
                                                    int frame_id, int* out_inst, int* out_prim, float* out_t,
@@ -1247,6 +1318,15 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
                 Bracket b(hook, "shade_normal", st);
                 hipLaunchKernelGGL(k_shade_normal, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
                                    nxt, B.freel[nxt]);
+            }
+        } else if (kind == Integrator::Direct) {
+            if constexpr (!BATCH) {
+                { Bracket b(hook, "shade_direct", st);
+                  hipLaunchKernelGGL(k_shade_direct, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins,
+                                     B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
+                { Bracket b(hook, "trace_shadow", st);
+                  launch_shadow_t<false>(count, B.stack_k, W, P, ctl, B.shq, nullptr, nullptr, B, L.refill,
+                                         L.grid_trace, st); }
             }
         } else if (kind == Integrator::Adjoint) {
             if constexpr (!BATCH) {

@@ -49,8 +49,10 @@ _SIGS = {
                          C.POINTER(abi.Camera), f32p, i32p], C.c_int),
     "bn_release_blas": ([C.c_void_p, C.c_int32], C.c_int),
     "bn_scene_export_instances": ([C.c_void_p, C.c_int32, C.POINTER(abi.InstanceUpload), i32p], C.c_int),
+    "bn_film_postprocess": ([f32p, C.c_int64, C.c_int32, f32p, C.POINTER(C.c_uint8)], C.c_int),
     "bn_render_pt": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p, C.POINTER(abi.TraceStats)], C.c_int),
     "bn_render_normal": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p], C.c_int),
+    "bn_render_direct": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), f32p], C.c_int),
     "bn_first_hit": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p], C.c_int),
     "bn_first_hit_noalloc": ([C.c_void_p, C.c_int32, C.POINTER(abi.PTParams), C.c_int32, i32p, i32p, f32p],
                              C.c_int),
@@ -240,6 +242,12 @@ class DeviceScene:
     def render_normal(self, params: abi.PTParams):
         film = np.zeros((params.width * params.height, 3), np.float32)
         check(self.ctx.lib.bn_render_normal(self.ctx.h, self.id, C.byref(params), _p(film, f32p)))
+        return film
+
+    def render_direct(self, params: abi.PTParams):
+        """DirectIntegrator (Direct.fs:10-40): Film.Pixels order, (H*W, 3)."""
+        film = np.zeros((params.width * params.height, 3), np.float32)
+        check(self.ctx.lib.bn_render_direct(self.ctx.h, self.id, C.byref(params), _p(film, f32p)))
         return film
 
     def first_hit(self, params: abi.PTParams, sample_id: int = 0):

@@ -270,6 +270,24 @@ int bn_scene_export_instances(bn_ctx* ctx, int32_t scene, bn_instance_upload* ou
 int bn_render_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
                  float* film_rgb, bn_trace_stats* stats);
 
+/* Film.PostProcess (Base/Film.fs:21-30): identity, ACES or gamma-2.2 tone
+ * curve per component, clamped to [0, 1] (out_rgb, 3 floats per pixel), and
+ * the 8-bit RGBA pixels Film.Save writes (Film.fs:55-66; out_rgba8, 4 bytes per
+ * pixel, v * 255 + 0.5 truncated, alpha 255, NaN -> 0), in Film.Pixels order
+ * (top row first).  Host only; either output may be NULL. */
+#define BN_TONE_IDENTITY 0
+#define BN_TONE_ACES     1
+#define BN_TONE_GAMMA    2
+int bn_film_postprocess(const float* film_rgb, int64_t n_pixels, int32_t tone_mapping,
+                        float* out_rgb, uint8_t* out_rgba8);
+
+/* DirectIntegrator (Extensions/Integrator/Direct.fs:10-40): emission seen
+ * directly, plus one light sample and one BSDF sample per camera hit, each
+ * divided by the light sampler's pdf (no MIS); max_depth and rr_depth are
+ * ignored, as the reference's DirectIntegrator(spp) has neither. */
+int bn_render_direct(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
+                     float* film_rgb);
+
 /* NormalIntegrator (Extensions/Integrator/Normal.fs:10-17), the first-hit view. */
 int bn_render_normal(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
                      float* film_rgb);
