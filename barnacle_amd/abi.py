@@ -29,7 +29,7 @@ class Instance(C.Structure):
 
 class Transform(C.Structure):
     _fields_ = [("flags", i32), ("scale", f32 * 3), ("rotation", f32 * 3),
-                ("translation", f32 * 3)]
+                ("translation", f32 * 3), ("time", f32), ("n_keys", i32)]
 
 
 class Node(C.Structure):
@@ -52,7 +52,7 @@ class SceneDesc(C.Structure):
                 ("transforms", C.POINTER(Transform)), ("n_transforms", i32),
                 ("nodes", C.POINTER(Node)), ("n_nodes", i32),
                 ("node_refs", C.POINTER(i32)), ("n_node_refs", i32),
-                ("root", i32), ("camera", Camera)]
+                ("root", i32), ("camera", Camera), ("time", f32)]
 
 
 class PTParams(C.Structure):

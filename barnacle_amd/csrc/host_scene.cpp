@@ -85,6 +85,154 @@ M4 keyframe_matrix(const bn_transform& t) {
     return S * R * T;
 }
 
+// ---------------------------------------------------- key-frame animation
+// Transform.Eval (Base/Scene.fs:27-36) and KeyFrame.Interpolate (:13-19) over
+// the System.Numerics routines they call (.NET 9 BCL, which the reference
+// does not vendor): Matrix4x4.Decompose, Quaternion.CreateFromRotationMatrix,
+// Quaternion.Slerp, Vector3.Lerp, and Transform.Compose with its argument
+// order as written (Util/Transform.fs:6-9): CreateTranslation(scale) *
+// CreateFromQuaternion(rotation) * CreateScale(translation).
+namespace {
+
+struct Quat { float x, y, z, w; };
+
+float det4(const M4& M) {                                 // Matrix4x4.GetDeterminant
+    const float* q = M.m;
+    const float a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5], g = q[6], h = q[7];
+    const float i = q[8], j = q[9], k = q[10], l = q[11], m = q[12], n = q[13], o = q[14], p = q[15];
+    const float kp_lo = k * p - l * o, jp_ln = j * p - l * n, jo_kn = j * o - k * n;
+    const float ip_lm = i * p - l * m, io_km = i * o - k * m, in_jm = i * n - j * m;
+    return a * (f * kp_lo - g * jp_ln + h * jo_kn) - b * (e * kp_lo - g * ip_lm + h * io_km) +
+           c * (e * jp_ln - f * ip_lm + h * in_jm) - d * (e * jo_kn - f * io_km + g * in_jm);
+}
+
+Quat quat_from_rotation(const M4& M) {                    // Quaternion.CreateFromRotationMatrix
+    const float* m = M.m;
+    const float m11 = m[0], m12 = m[1], m13 = m[2], m21 = m[4], m22 = m[5], m23 = m[6];
+    const float m31 = m[8], m32 = m[9], m33 = m[10];
+    const float trace = m11 + m22 + m33;
+    Quat q;
+    if (trace > 0.0f) {
+        float s = std::sqrt(trace + 1.0f);
+        q.w = s * 0.5f;
+        s = 0.5f / s;
+        q.x = (m23 - m32) * s; q.y = (m31 - m13) * s; q.z = (m12 - m21) * s;
+    } else if (m11 >= m22 && m11 >= m33) {
+        const float s = std::sqrt(1.0f + m11 - m22 - m33), inv = 0.5f / s;
+        q.x = 0.5f * s; q.y = (m12 + m21) * inv; q.z = (m13 + m31) * inv; q.w = (m23 - m32) * inv;
+    } else if (m22 > m33) {
+        const float s = std::sqrt(1.0f + m22 - m11 - m33), inv = 0.5f / s;
+        q.x = (m21 + m12) * inv; q.y = 0.5f * s; q.z = (m32 + m23) * inv; q.w = (m31 - m13) * inv;
+    } else {
+        const float s = std::sqrt(1.0f + m33 - m11 - m22), inv = 0.5f / s;
+        q.x = (m31 + m13) * inv; q.y = (m32 + m23) * inv; q.z = 0.5f * s; q.w = (m12 - m21) * inv;
+    }
+    return q;
+}
+
+// Matrix4x4.Decompose: scales are the lengths of the basis rows; the rows are
+// normalised largest scale first, a degenerate row (scale < 1e-4) replaced by
+// a canonical axis or a cross product; a left-handed basis negates the
+// largest scale; a basis whose determinant is not 1 (within 1e-4 after
+// squaring the difference) gives the identity rotation.
+void decompose(const M4& M, V3& scale, Quat& rot, V3& trans) {
+    constexpr float eps = 1e-4f;
+    const V3 canon[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    trans = {M.m[12], M.m[13], M.m[14]};
+    V3 row[3] = {{M.m[0], M.m[1], M.m[2]}, {M.m[4], M.m[5], M.m[6]}, {M.m[8], M.m[9], M.m[10]}};
+    float sc[3] = {length(row[0]), length(row[1]), length(row[2])};
+    int a, b, c;
+    const float x = sc[0], y = sc[1], z = sc[2];
+    if (x < y) {
+        if (y < z) { a = 2; b = 1; c = 0; }
+        else { a = 1; if (x < z) { b = 2; c = 0; } else { b = 0; c = 2; } }
+    } else {
+        if (x < z) { a = 2; b = 0; c = 1; }
+        else { a = 0; if (y < z) { b = 2; c = 1; } else { b = 1; c = 2; } }
+    }
+    if (sc[a] < eps) row[a] = canon[a];
+    row[a] = normalize(row[a]);
+    if (sc[b] < eps) {
+        const float ax = std::fabs(row[a].x), ay = std::fabs(row[a].y), az = std::fabs(row[a].z);
+        int cc;
+        if (ax < ay) cc = (ay < az) ? 0 : (ax < az ? 0 : 2);
+        else cc = (ax < az) ? 1 : (ay < az ? 1 : 2);
+        row[b] = cross(row[a], canon[cc]);
+    }
+    row[b] = normalize(row[b]);
+    if (sc[c] < eps) row[c] = cross(row[a], row[b]);
+    row[c] = normalize(row[c]);
+    M4 basis = M4::identity();
+    for (int r = 0; r < 3; ++r) { basis.m[4 * r] = row[r].x; basis.m[4 * r + 1] = row[r].y; basis.m[4 * r + 2] = row[r].z; }
+    float det = det4(basis);
+    if (det < 0.0f) {
+        sc[a] = -sc[a];
+        row[a] = -row[a];
+        basis.m[4 * a] = row[a].x; basis.m[4 * a + 1] = row[a].y; basis.m[4 * a + 2] = row[a].z;
+        det = -det;
+    }
+    det -= 1.0f;
+    det *= det;
+    rot = eps < det ? Quat{0, 0, 0, 1} : quat_from_rotation(basis);
+    scale = {sc[0], sc[1], sc[2]};
+}
+
+Quat slerp(Quat q1, Quat q2, float t) {                   // Quaternion.Slerp
+    constexpr float eps = 1e-6f;
+    float cosw = q1.x * q2.x + q1.y * q2.y + q1.z * q2.z + q1.w * q2.w;
+    bool flip = false;
+    if (cosw < 0.0f) { flip = true; cosw = -cosw; }
+    float s1, s2;
+    if (cosw > 1.0f - eps) {
+        s1 = 1.0f - t;
+        s2 = flip ? -t : t;
+    } else {
+        const float w = (float)std::acos((double)cosw);
+        const float inv = 1.0f / (float)std::sin((double)w);
+        s1 = (float)std::sin((double)((1.0f - t) * w)) * inv;
+        const float b = (float)std::sin((double)(t * w)) * inv;
+        s2 = flip ? -b : b;
+    }
+    return {q1.x * s1 + q2.x * s2, q1.y * s1 + q2.y * s2, q1.z * s1 + q2.z * s2, q1.w * s1 + q2.w * s2};
+}
+
+V3 lerp(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }   // Vector3.Lerp
+
+M4 from_quat(Quat q) {                                    // Matrix4x4.CreateFromQuaternion
+    const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    const float xy = q.x * q.y, wz = q.z * q.w, xz = q.z * q.x, wy = q.y * q.w, yz = q.y * q.z, wx = q.x * q.w;
+    M4 r = M4::identity();
+    r.m[0] = 1.0f - 2.0f * (yy + zz); r.m[1] = 2.0f * (xy + wz); r.m[2] = 2.0f * (xz - wy);
+    r.m[4] = 2.0f * (xy - wz); r.m[5] = 1.0f - 2.0f * (zz + xx); r.m[6] = 2.0f * (yz + wx);
+    r.m[8] = 2.0f * (xz + wy); r.m[9] = 2.0f * (yz - wx); r.m[10] = 1.0f - 2.0f * (yy + xx);
+    return r;
+}
+
+}  // namespace
+
+M4 transform_eval(const bn_transform* keys, int n, float time) {
+    if (n <= 0) throw Error(BN_E_INVALID, "KeyFrames cannot be empty");
+    int prev = -1;                                        // Array.tryFindIndexBack (k.Time <= t)
+    for (int i = n - 1; i >= 0; --i)
+        if (keys[i].time <= time) { prev = i; break; }
+    if (prev < 0) return keyframe_matrix(keys[0]);
+    if (prev == n - 1) return keyframe_matrix(keys[prev]);
+    const bn_transform& A = keys[prev];
+    const bn_transform& B = keys[prev + 1];
+    const float ratio = (time - A.time) / (B.time - A.time);
+    V3 s1, t1, s2, t2;
+    Quat r1, r2;
+    decompose(keyframe_matrix(A), s1, r1, t1);
+    decompose(keyframe_matrix(B), s2, r2, t2);
+    const V3 tr = lerp(t1, t2, ratio);
+    const Quat rq = slerp(r1, r2, ratio);
+    const V3 sc = lerp(s1, s2, ratio);
+    M4 T = M4::identity(), S = M4::identity();            // Compose(scale, rotation, translation)
+    T.m[12] = sc.x; T.m[13] = sc.y; T.m[14] = sc.z;       // CreateTranslation(scale)
+    S.m[0] = tr.x; S.m[5] = tr.y; S.m[10] = tr.z;         // CreateScale(translation)
+    return T * from_quat(rq) * S;
+}
+
 Box transform_box(const Box& b, const M4& m) {
     Box r = Box::empty();
     for (int c = 0; c < 8; ++c) {
@@ -281,7 +429,13 @@ void build_scene(const bn_scene_desc& d, HostScene& S) {
         m.alias = build_alias(area.data(), nt);
     }
     std::vector<M4> local(d.n_transforms);
-    for (int i = 0; i < d.n_transforms; ++i) local[i] = keyframe_matrix(d.transforms[i]);
+    // A transform is a run of key frame records sorted by time (Loader.fs:47);
+    // nodes reference its first record, whose n_keys gives the run length.
+    for (int i = 0; i < d.n_transforms; ++i) {
+        const int nk = d.transforms[i].n_keys > 0 ? d.transforms[i].n_keys : 1;
+        if (i + nk > d.n_transforms) throw Error(BN_E_INVALID, "key frame run past the transform array");
+        local[i] = transform_eval(d.transforms + i, nk, d.time);
+    }
 
     // Node.Traverse: DFS, world = local * parent.  An instance object listed
     // by several nodes keeps the transform of its last visit for every entry.

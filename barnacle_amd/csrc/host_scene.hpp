@@ -66,6 +66,7 @@ V3 transform_point(V3 v, const M4& m);          // Vector3.Transform
 inline V3 transform_dir(V3 v, const M4& m) { return transform_point(v, m) - m.translation(); }
 M4 invert(const M4& m);                          // Matrix4x4.Invert
 M4 keyframe_matrix(const bn_transform& t);       // KeyFrameInfo.ToKeyFrame
+M4 transform_eval(const bn_transform* keys, int n, float time);   // Transform.Eval
 
 struct Box {
     V3 lo, hi;

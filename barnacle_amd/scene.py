@@ -98,6 +98,7 @@ class Scene:
         self.instances, self.transforms, self.nodes = [], [], []
         self.node_refs = []
         self.root = 0
+        self.time = np.float32(0.0)           # Scene.Render(t): the reference renders t = 0
         self.camera = None
         self.integrator = {}
         self.film = (0, 0, "aces")
@@ -144,7 +145,7 @@ class Scene:
         insts, n_i = arr(abi.Instance, self.instances, lambda i: abi.Instance(*i))
         xfs, n_x = arr(abi.Transform, self.transforms, lambda t: abi.Transform(
             t["flags"], (C.c_float * 3)(*t["scale"]), (C.c_float * 3)(*t["rotation"]),
-            (C.c_float * 3)(*t["translation"])))
+            (C.c_float * 3)(*t["translation"]), t["time"], t.get("n_keys", 0)))
         nodes, n_n = arr(abi.Node, self.nodes, lambda n: abi.Node(*n))
         refs = np.asarray(self.node_refs if self.node_refs else [0], np.int32)
         keep.append(refs)
@@ -155,7 +156,7 @@ class Scene:
             prims, n_p, mats, n_m, lights, n_l, insts, n_i, xfs, n_x, nodes, n_n,
             refs.ctypes.data_as(C.POINTER(C.c_int32)), len(self.node_refs), self.root,
             abi.Camera(cam["kind"], cam["fov"], cam["aspect"], cam["aperture"],
-                       cam["focus_distance"], cam["push_forward"]))
+                       cam["focus_distance"], cam["push_forward"]), float(self.time))
         return d, keep
 
 
@@ -177,20 +178,17 @@ def _keyframe(kf) -> dict:
             "translation": [np.float32(x) for x in t[:3]]}
 
 
-def _eval_transform(keyframes) -> dict:
-    """Transform.Eval(0) (Base/Scene.fs:27-36) over key frames sorted by time
-    (stable, Loader.fs:47).  Interpolating between two key frames is
-    keyframe animation, which this path does not carry."""
+def _keyframes(keyframes) -> list:
+    """TransformInfo.ToTransform (Loader.fs:44-47): the key frames sorted by
+    time.  Array.sortBy is not stable, so the reference orders equal times
+    arbitrarily; here they keep their order in the file.  The native scene
+    build evaluates them with Transform.Eval (Base/Scene.fs:27-36),
+    interpolating between key frames (:13-19)."""
     if not keyframes:
         raise SceneError("KeyFrames cannot be empty")
     ks = sorted((_keyframe(k) for k in keyframes), key=lambda k: k["time"])
-    prev = None
-    for i, k in enumerate(ks):
-        if k["time"] <= 0:
-            prev = i
-    if prev is None or prev == len(ks) - 1:
-        return ks[prev if prev is not None else 0]
-    raise SceneError("key-frame interpolation at t=0 (Base/Scene.fs:13-19) is not supported")
+    ks[0]["n_keys"] = len(ks)
+    return ks
 
 
 def load_scene(src, base_dir: str | None = None) -> Scene:
@@ -263,8 +261,10 @@ def load_scene(src, base_dir: str | None = None) -> Scene:
         if not (0 <= p < n_prim) or m >= n_mat or l >= n_light or m < -1 or l < -1:
             raise SceneError("instance references an undefined primitive/material/light")
         sc.instances.append((int(p), m, l))
+    first_key = []
     for t in info.get("transforms") or []:
-        sc.transforms.append(_eval_transform(t.get("keyframes") or []))
+        first_key.append(len(sc.transforms))
+        sc.transforms += _keyframes(t.get("keyframes") or [])
 
     nodes = info.get("nodes") or []
     for n in nodes:
@@ -273,8 +273,9 @@ def load_scene(src, base_dir: str | None = None) -> Scene:
         if any(not (0 <= k < len(sc.instances)) for k in inst) or any(not (0 <= k < len(nodes)) for k in ch):
             raise SceneError("node references an undefined instance/node")
         tr = -1 if n.get("transform") is None else int(n["transform"])
-        if tr >= len(sc.transforms):
+        if tr >= len(first_key) or tr < -1:
             raise SceneError("node references an undefined transform")
+        tr = first_key[tr] if tr >= 0 else -1
         io = len(sc.node_refs)
         sc.node_refs += inst
         co = len(sc.node_refs)

@@ -86,11 +86,18 @@ typedef struct bn_instance {           /* PrimitiveInstanceInfo (:118-145)   */
 #define BN_XF_ROTATION     2
 #define BN_XF_TRANSLATION  4
 
-typedef struct bn_transform {          /* the key frame Transform.Eval(0)    */
+/* One key frame (KeyFrameInfo, Loader.fs:14-41).  A transform is a run of
+ * n_keys consecutive records sorted by time (Loader.fs:47); nodes reference
+ * the run's first record, which carries n_keys (0 counts as 1).  The scene is
+ * evaluated at bn_scene_desc.time by Transform.Eval (Base/Scene.fs:27-36),
+ * interpolating between key frames as KeyFrame.Interpolate does (:13-19). */
+typedef struct bn_transform {
     int32_t flags;                     /* BN_XF_* present components         */
     float   scale[3];                  /* (Loader.fs:20-41): S*Rx*Ry*Rz*T    */
     float   rotation[3];               /* radians                            */
     float   translation[3];
+    float   time;                      /* key frame time, default 0          */
+    int32_t n_keys;                    /* key frames in this run (first)     */
 } bn_transform;
 
 typedef struct bn_node {               /* NodeInfo (Loader.fs:147-168)       */
@@ -126,6 +133,7 @@ typedef struct bn_scene_desc {
     const int32_t*      node_refs;    int32_t n_node_refs;
     int32_t             root;
     bn_camera           camera;
+    float               time;         /* Scene.Render(t): the reference renders t = 0 */
 } bn_scene_desc;
 
 /* ------------------------------------------------------ render parameters */

@@ -1119,6 +1119,134 @@ MLTPath mltPath(const orc_scene& s, const orc_mlt_params& p, MLTSampler& m, Stat
     return {L, luminance(L), ix, iy};
 }
 
+// ------------------------------------------------------------ key frames
+// KeyFrameInfo.ToKeyFrame (Loader.fs:20-41): S * Rx * Ry * Rz * T.
+M4 keyFrameMatrix(const orc_transform& t) {
+    M4 S = (t.flags & 1) ? scaleM(t.scale[0], t.scale[1], t.scale[2]) : identity();
+    M4 R = (t.flags & 2) ? mul(mul(rotX(t.rotation[0]), rotY(t.rotation[1])), rotZ(t.rotation[2])) : identity();
+    M4 T = (t.flags & 4) ? translateM(t.translation[0], t.translation[1], t.translation[2]) : identity();
+    return mul(mul(S, R), T);
+}
+struct Q4 { float x, y, z, w; };
+// Quaternion.CreateFromRotationMatrix (System.Numerics): the largest of the
+// trace and the diagonal picks the component computed from a square root.
+Q4 rotToQuat(const float* m) {
+    const float tr = m[0] + m[5] + m[10];
+    if (tr > 0.0f) {
+        float s = std::sqrt(tr + 1.0f);
+        float w = s * 0.5f;
+        s = 0.5f / s;
+        return {(m[6] - m[9]) * s, (m[8] - m[2]) * s, (m[1] - m[4]) * s, w};
+    }
+    if (m[0] >= m[5] && m[0] >= m[10]) {
+        float s = std::sqrt(1.0f + m[0] - m[5] - m[10]);
+        float k = 0.5f / s;
+        return {0.5f * s, (m[1] + m[4]) * k, (m[2] + m[8]) * k, (m[6] - m[9]) * k};
+    }
+    if (m[5] > m[10]) {
+        float s = std::sqrt(1.0f + m[5] - m[0] - m[10]);
+        float k = 0.5f / s;
+        return {(m[4] + m[1]) * k, 0.5f * s, (m[9] + m[6]) * k, (m[8] - m[2]) * k};
+    }
+    float s = std::sqrt(1.0f + m[10] - m[0] - m[5]);
+    float k = 0.5f / s;
+    return {(m[8] + m[2]) * k, (m[9] + m[6]) * k, 0.5f * s, (m[1] - m[4]) * k};
+}
+// Matrix4x4.GetDeterminant of a matrix whose fourth row and column are those
+// of the identity (the only kind Decompose builds): its cofactor expansion
+// reduces to the 3 x 3 one below (the dropped terms add zeros).
+float det3of4(const float* q) {
+    return q[0] * (q[5] * q[10] - q[6] * q[9]) - q[1] * (q[4] * q[10] - q[6] * q[8]) +
+           q[2] * (q[4] * q[9] - q[5] * q[8]);
+}
+// Matrix4x4.Decompose (System.Numerics).
+void decomposeM(const M4& M, V3& sc, Q4& q, V3& tr) {
+    const float EPS = 1e-4f;
+    tr = {M.m[12], M.m[13], M.m[14]};
+    V3 r[3];
+    for (int i = 0; i < 3; ++i) r[i] = {M.m[4 * i], M.m[4 * i + 1], M.m[4 * i + 2]};
+    float s[3] = {length(r[0]), length(r[1]), length(r[2])};
+    int a, b, c;                                               // ranking, largest first
+    if (s[0] < s[1]) {
+        if (s[1] < s[2]) { a = 2; b = 1; c = 0; } else if (s[0] < s[2]) { a = 1; b = 2; c = 0; } else { a = 1; b = 0; c = 2; }
+    } else {
+        if (s[0] < s[2]) { a = 2; b = 0; c = 1; } else if (s[1] < s[2]) { a = 0; b = 2; c = 1; } else { a = 0; b = 1; c = 2; }
+    }
+    const V3 axes[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (s[a] < EPS) r[a] = axes[a];
+    r[a] = normalize(r[a]);
+    if (s[b] < EPS) {
+        float fx = std::fabs(r[a].x), fy = std::fabs(r[a].y), fz = std::fabs(r[a].z);
+        int cc;
+        if (fx < fy) { if (fy < fz) cc = 0; else if (fx < fz) cc = 0; else cc = 2; }
+        else { if (fx < fz) cc = 1; else if (fy < fz) cc = 1; else cc = 2; }
+        r[b] = cross(r[a], axes[cc]);
+    }
+    r[b] = normalize(r[b]);
+    if (s[c] < EPS) r[c] = cross(r[a], r[b]);
+    r[c] = normalize(r[c]);
+    float basis[16] = {r[0].x, r[0].y, r[0].z, 0, r[1].x, r[1].y, r[1].z, 0, r[2].x, r[2].y, r[2].z, 0, 0, 0, 0, 1};
+    float det = det3of4(basis);
+    if (det < 0.0f) {
+        s[a] = -s[a];
+        r[a] = -1.0f * r[a];
+        basis[4 * a] = r[a].x; basis[4 * a + 1] = r[a].y; basis[4 * a + 2] = r[a].z;
+        det = -det;
+    }
+    det = det - 1.0f;
+    det = det * det;
+    q = (EPS < det) ? Q4{0, 0, 0, 1} : rotToQuat(basis);
+    sc = {s[0], s[1], s[2]};
+}
+// Quaternion.Slerp (System.Numerics), epsilon 1e-6.
+Q4 slerpQ(Q4 p, Q4 q, float t) {
+    float co = p.x * q.x + p.y * q.y + p.z * q.z + p.w * q.w;
+    bool neg = co < 0.0f;
+    if (neg) co = -co;
+    float a, b;
+    if (co > 1.0f - 1e-6f) {
+        a = 1.0f - t;
+        b = neg ? -t : t;
+    } else {
+        float om = (float)std::acos((double)co);
+        float inv = 1.0f / sinF(om);
+        a = sinF((1.0f - t) * om) * inv;
+        b = sinF(t * om) * inv;
+        if (neg) b = -b;
+    }
+    return {p.x * a + q.x * b, p.y * a + q.y * b, p.z * a + q.z * b, p.w * a + q.w * b};
+}
+// Matrix4x4.CreateFromQuaternion (System.Numerics).
+M4 quatToM(Q4 q) {
+    M4 r = identity();
+    float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    float xy = q.x * q.y, wz = q.z * q.w, xz = q.z * q.x, wy = q.y * q.w, yz = q.y * q.z, wx = q.x * q.w;
+    r.m[0] = 1.0f - 2.0f * (yy + zz); r.m[1] = 2.0f * (xy + wz); r.m[2] = 2.0f * (xz - wy);
+    r.m[4] = 2.0f * (xy - wz); r.m[5] = 1.0f - 2.0f * (zz + xx); r.m[6] = 2.0f * (yz + wx);
+    r.m[8] = 2.0f * (xz + wy); r.m[9] = 2.0f * (yz - wx); r.m[10] = 1.0f - 2.0f * (yy + xx);
+    return r;
+}
+// Transform.Eval (Base/Scene.fs:27-36) with KeyFrame.Interpolate (:13-19):
+// decompose both key frames, Lerp (Vector3.Lerp: a (1 - t) + b t) and Slerp,
+// then Transform.Compose (Util/Transform.fs:6-9) as written:
+// CreateTranslation(scale) * CreateFromQuaternion(rotation) * CreateScale(translation).
+M4 evalTransform(const orc_transform* k, int n, float t) {
+    int prev = -1;
+    for (int i = 0; i < n; ++i)
+        if (k[i].time <= t) prev = i;
+    if (prev < 0) return keyFrameMatrix(k[0]);
+    if (prev == n - 1) return keyFrameMatrix(k[prev]);
+    float ratio = (t - k[prev].time) / (k[prev + 1].time - k[prev].time);
+    V3 s1, t1, s2, t2;
+    Q4 q1, q2;
+    decomposeM(keyFrameMatrix(k[prev]), s1, q1, t1);
+    decomposeM(keyFrameMatrix(k[prev + 1]), s2, q2, t2);
+    auto lerp3 = [](V3 a, V3 b, float u) { return a * (1.0f - u) + b * u; };
+    V3 tr = lerp3(t1, t2, ratio), sc = lerp3(s1, s2, ratio);
+    Q4 q = slerpQ(q1, q2, ratio);
+    return mul(mul(translateM(sc.x, sc.y, sc.z), quatToM(q)), scaleM(tr.x, tr.y, tr.z));
+}
+
 int desc_build(const orc_scene_desc* d, orc_scene* s);
 
 }  // namespace
@@ -1744,12 +1872,10 @@ int desc_build(const orc_scene_desc* d, orc_scene* s) {
         s->meshes.push_back(std::move(m));
     }
     std::vector<M4> locals(d->n_transforms);
-    for (int i = 0; i < d->n_transforms; ++i) {               // KeyFrameInfo.ToKeyFrame
-        const orc_transform& t = d->transforms[i];
-        M4 S = (t.flags & 1) ? scaleM(t.scale[0], t.scale[1], t.scale[2]) : identity();
-        M4 R = (t.flags & 2) ? mul(mul(rotX(t.rotation[0]), rotY(t.rotation[1])), rotZ(t.rotation[2])) : identity();
-        M4 T = (t.flags & 4) ? translateM(t.translation[0], t.translation[1], t.translation[2]) : identity();
-        locals[i] = mul(mul(S, R), T);
+    for (int i = 0; i < d->n_transforms; ++i) {               // Transform.Eval (Base/Scene.fs:27-36)
+        const int nk = d->transforms[i].n_keys > 0 ? d->transforms[i].n_keys : 1;
+        if (i + nk > d->n_transforms) return fail("key frame run past the transform array");
+        locals[i] = evalTransform(d->transforms + i, nk, d->time);
     }
     // Node.Traverse (Base/Scene.fs:43-51).  Instances are shared objects: the
     // last UpdateTransform of an instance wins for every list entry of it.

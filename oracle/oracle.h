@@ -29,7 +29,7 @@ typedef struct { int32_t kind, vert_offset, n_verts, index_offset, n_indices; fl
 typedef struct { int32_t kind; float albedo[3]; float ior, roughness, metallic; } orc_material;
 typedef struct { int32_t kind; float emission[3]; int32_t two_sided; } orc_light;
 typedef struct { int32_t primitive, material, light; } orc_instance;
-typedef struct { int32_t flags; float scale[3], rotation[3], translation[3]; } orc_transform;
+typedef struct { int32_t flags; float scale[3], rotation[3], translation[3]; float time; int32_t n_keys; } orc_transform;
 typedef struct { int32_t transform, has_camera, inst_offset, n_inst, child_offset, n_child; } orc_node;
 typedef struct { int32_t kind; float fov, aspect, aperture, focus_distance, push_forward; } orc_camera;
 typedef struct {
@@ -44,6 +44,7 @@ typedef struct {
     const int32_t* node_refs;         int32_t n_node_refs;
     int32_t root;
     orc_camera camera;
+    float time;
 } orc_scene_desc;
 
 typedef struct { int32_t width, height, spp, max_depth, rr_depth, frame_id; } orc_pt_params;

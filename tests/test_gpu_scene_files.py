@@ -216,3 +216,34 @@ def test_cbox_pssmlt_against_sample_render(ctx):
           f"the sphere {outside.max():.4f}, on it {d[SPHERE].max():.4f}", flush=True)
     assert np.all(np.abs(mean - ref) <= 0.01)
     assert outside.max() <= 0.05
+
+
+def test_keyframe_animation_matches_oracle(ctx, orc):
+    """Scenes whose transforms interpolate between key frames at t = 0: the
+    scene build's instance matrices equal the oracle's bit for bit, and so do
+    the first hits of the animated sphere."""
+    import test_keyframes as kf
+    cases = [
+        [{"time": 1.0, "scale": [4.0, 4.0, 4.0], "rotation": [0.0, 1.0, 0.0], "translation": [30.0, 4.0, 5.0]},
+         {"time": -1.0, "scale": [2.0, 2.0, 2.0], "rotation": [0.0, 0.4, 0.0], "translation": [10.0, 2.0, 3.0]}],
+        [{"time": 0.0, "scale": [2.0, 2.0, 2.0], "rotation": [0.3, 0.0, 0.0], "translation": [5.0, 6.0, 7.0]},
+         {"time": 1.0, "scale": [9.0, 9.0, 9.0], "translation": [1.0, 1.0, 1.0]}],
+        [{"time": -2.0, "rotation": [0.0, 3.0, 0.0], "translation": [8.0, 9.0, 10.0]},
+         {"time": 2.0, "rotation": [0.0, -3.0, 0.0], "translation": [8.0, 9.0, 10.0]}],
+    ]
+    for kfs in cases:
+        info = kf.animated_scene(kfs)
+        info["film"]["width"] = info["film"]["height"] = 64
+        sc = scene.load_scene(info)
+        ds = ctx.scene(sc)
+        g, o = ds.export(), orc.scene(sc).export()
+        assert np.array_equal(g["inst_order"], o["inst_order"])
+        assert np.array_equal(g["o2w"].view(np.int32), o["o2w"].view(np.int32))
+        assert np.array_equal(g["w2o"].view(np.int32), o["w2o"].view(np.int32))
+        p = sc.pt_params()
+        gi, gp, gt = ds.first_hit(p, 0)
+        oi, op, ot = orc.scene(sc).first_hit(p, 0)
+        assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+        assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
+        k = int(np.nonzero(o["inst_order"] == 0)[0][0])
+        assert np.sum(gi == k) >= 4                           # the animated sphere is in view
