@@ -16,8 +16,13 @@ already resident in HBM.  Prints one JSON line:
                  (forward + path-replay backward) per second, and their time
                  over the forward render's
   pssmlt         the caustic scene: mutations per second and acceptance
+  rooflines      the same for trace_shadow and shade_c (192 B per path
+                 vertex), each with its L2 hit rate (committed TCC_HIT / TCC_MISS
+                 summary) and measured-traffic fraction
   cpu_baseline   the CPU oracle (C++ restatement of the F# renderer) on a
-                 bounded pixel window of the same frame, all host threads up to 16
+                 bounded pixel window of the same frame, on one GPU's share of
+                 the host (16 threads), with the CPU model; plus bounded CPU
+                 legs of PSSMLT (config 4) and the adjoint (config 3)
   parity         GPU vs oracle on a 128x128x2spp render of the same scene
 
 --copy-calibration additionally copies 1 GiB device to device (hipMemcpy) so
@@ -46,9 +51,35 @@ def closest_bytes(st):
     return 48 * st["rays_closest"] + 32 * st["nodes_closest"] + 36 * st["tris_closest"] + 48 * st["insts_closest"]
 
 
+def shadow_bytes(st):
+    return 36 * st["rays_shadow"] + 32 * st["nodes_shadow"] + 36 * st["tris_shadow"] + 48 * st["insts_shadow"]
+
+
+def shade_bytes(st):
+    return 192 * st["vertices"]
+
+
 def frame_bytes(st):
-    shadow = 36 * st["rays_shadow"] + 32 * st["nodes_shadow"] + 36 * st["tris_shadow"] + 48 * st["insts_shadow"]
-    return closest_bytes(st) + shadow + 192 * st["vertices"] + 12 * st["paths"]
+    return closest_bytes(st) + shadow_bytes(st) + shade_bytes(st) + 12 * st["paths"]
+
+
+def host_cpu():
+    """Host threads this process may use and the CPU model (/proc/cpuinfo)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    # One GPU's share of the GPU box's host is 16 threads; use all of them.
+    return {"threads": min(16, avail), "available_threads": avail, "cpu_count": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
@@ -57,7 +88,8 @@ def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
     parity check: rel-RMSE and the bit-equal fraction over their pixels."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    threads = cpu["threads"]
     osc = oracle.Oracle().scene(sc)
     W, H = params.width, params.height
     tiles = sorted(((x, y) for y in range(0, H, 64) for x in range(0, W, 64)),
@@ -79,7 +111,7 @@ def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
         n += 1
         if elapsed >= budget_s:
             break
-    out = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+    out = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port", "host": cpu,
            "sample": f"{n} 64x64-pixel windows at the film centre of the same 1024x1024x64spp frame "
                      f"({paths} paths, {rays} rays, {elapsed:.1f} s); oracle = C++ restatement of the F# "
                      f"renderer (no .NET runtime available)",
@@ -197,6 +229,52 @@ def pssmlt(ctx):
             "nan_pixel_fraction": float((~np.isfinite(film).all(axis=1)).mean())}
 
 
+def cpu_pssmlt_and_adjoint(budget_s=8.0):
+    """Bounded CPU legs of the other two workloads, on the oracle: PSSMLT
+    mutations/s on a chain subsample of config 4 (its whole bootstrap, then
+    chains [0, n)), and path-replay adjoint paths/s on a pixel window of
+    config 3 (the FP64 oracle adjoint of the same definition)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = host_cpu()["threads"]
+    orc = oracle.Oracle()
+    out = {"cores": threads}
+    sc = scene.load_scene(configs.caustic_scene())
+    osc = orc.scene(sc)
+    p = sc.mlt_params(n_bootstrap=1 << 16)
+    t0 = time.perf_counter()
+    osc.render_pssmlt(p, threads=threads, chains=(0, 1))         # bootstrap + one chain
+    boot_s = time.perf_counter() - t0
+    n_ch = 4096
+    t0 = time.perf_counter()
+    _, st, _, _ = osc.render_pssmlt(p, threads=threads, chains=(0, n_ch))
+    el = time.perf_counter() - t0 - boot_s
+    muts = n_ch * st["mutations_per_chain"]
+    out["pssmlt"] = {"value": muts / el / 1e6, "unit": "Mmutations/s",
+                     "sample": f"config4 with 2^16 bootstrap paths, chains 0..{n_ch} of 2^20 x "
+                               f"{st['mutations_per_chain']} mutations ({muts} mutations in {el:.1f} s, "
+                               f"bootstrap time {boot_s:.2f} s subtracted)"}
+    info = configs.config3()
+    sc3 = scene.load_scene(info)
+    osc3 = orc.scene(sc3)
+    p3 = sc3.pt_params()
+    d = np.random.default_rng(0).normal(size=(p3.width * p3.height, 3)).astype(np.float32)
+    side = 16
+    while True:
+        x0 = y0 = 256 - side // 2
+        t0 = time.perf_counter()
+        osc3.adjoint_pt(p3, d, threads=threads, window=(x0, y0, x0 + side, y0 + side))
+        el = time.perf_counter() - t0
+        if el > budget_s / 4 or side >= 128:
+            break
+        side *= 2
+    paths = side * side * p3.spp
+    out["adjoint"] = {"value": paths / el / 1e6, "unit": "Mpaths/s (forward replay + FP64 adjoint)",
+                      "sample": f"config3, a {side}x{side} pixel window at the film centre, {p3.spp} spp "
+                                f"({paths} paths, {el:.1f} s)"}
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device hipMemcpy (the runtime's copy kernel) of nbytes."""
     import ctypes
@@ -221,7 +299,7 @@ def parity(ctx):
     sc = scene.load_scene(configs.config1(width=128, height=128, spp=2))
     p = sc.pt_params()
     g = ctx.scene(sc).render_pt(p)
-    o, _ = oracle.Oracle().scene(sc).render(p, threads=min(16, os.cpu_count() or 1))
+    o, _ = oracle.Oracle().scene(sc).render(p, threads=host_cpu()["threads"])
     rel = float(np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)))
     out = {"config": "same scene at 128x128x2spp", "rel_rmse": rel, "bit_equal_fraction": float(np.mean(g == o)),
            "tolerance": 1e-3}
@@ -233,7 +311,7 @@ def parity(ctx):
     t0 = time.perf_counter()
     g0 = ds0.render_pt(p0)
     gpu_s = time.perf_counter() - t0
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpu()["threads"]
     t0 = time.perf_counter()
     o0, st0 = oracle.Oracle().scene(sc0).render(p0, threads=threads)
     cpu_s = time.perf_counter() - t0
@@ -289,17 +367,41 @@ def main():
     ctx.set_option("timing", 0)
 
     ms_per_step = elapsed / args.steps * 1e3
-    launches, total_ms = kt["trace_closest"]
-    per_launch_ms = total_ms / launches
-    launches_per_frame = launches / args.steps
-    bytes_per_launch = closest_bytes(st) / launches_per_frame
-    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
 
-    traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", "trace_closest_config1_pmc.json")
-    if os.path.exists(pmc_path):
-        with open(pmc_path) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    # Committed rocprofv3 summaries of the same workload: HBM bytes per
+    # launch (FETCH_SIZE / WRITE_SIZE) and L2 hit rates (TCC_HIT / TCC_MISS).
+    def load_profile(name):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            return {}
+        with open(path) as fh:
+            return json.load(fh)
+    pmc = load_profile("trace_closest_config1_pmc.json")
+    l2 = load_profile("l2_hit_config1.json").get("kernels", {})
+    traffic_of = {"trace_closest": pmc.get("hbm_bytes_per_launch"),
+                  "trace_shadow": pmc.get("trace_shadow", {}).get("hbm_bytes_per_launch"),
+                  "shade_c": pmc.get("shade", {}).get("hbm_bytes_per_launch")}
+
+    def roofline(kernel, timer_name, frame_algorithmic):
+        launches, total_ms = kt[timer_name]
+        per_launch_ms = total_ms / launches
+        bytes_per_launch = frame_algorithmic / (launches / args.steps)
+        achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+        traffic = traffic_of.get(kernel)
+        r = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": per_launch_ms,
+             "launches": launches, "l2_hit_rate": l2.get(kernel, {}).get("l2_hit_rate")}
+        if traffic:
+            # what the hardware moved: measured HBM bytes over the same launch time
+            r["traffic_gbs"] = traffic / (per_launch_ms * 1e-3) / 1e9
+            r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
+        return r
+    rl_closest = roofline("trace_closest", "trace_closest", closest_bytes(st))
+    # The traversal kernels keep 85% (trace_closest) of their VALU issue slots
+    # busy (profiles/trace_config1_sq_counters.md): the byte roofline is the
+    # contract's yardstick, instruction issue is what limits them.
+    rl_closest["limiter"] = "VALU instruction issue (85% of issue slots, profiles/trace_config1_sq_counters.md)"
 
     out = {
         "metric": "Mrays/s (batched vs plain, %HBM peak) + adjoint iters/s at RMSE<=1e-3 vs CPU",
@@ -321,10 +423,9 @@ def main():
         "rays_per_frame": {"extension": st["rays_closest"], "shadow": st["rays_shadow"]},
         "frame_algorithmic_bytes": frame_bytes(st),
         "frame_hbm_roofline_frac": frame_bytes(st) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "roofline": {"kernel": "trace_closest", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": per_launch_ms,
-                     "launches": launches},
+        "roofline": rl_closest,
+        "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st)),
+                      "shade_c": roofline("shade_c", "shade", shade_bytes(st))},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in kt.items()},
     }
     if not args.no_batch:
@@ -339,6 +440,7 @@ def main():
         copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, params, film)
+        out["cpu_baseline"]["other_workloads"] = cpu_pssmlt_and_adjoint()
     print(json.dumps(out), flush=True)
 
 
