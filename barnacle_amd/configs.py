@@ -227,40 +227,76 @@ def split_mesh(mesh, n_patches):
     return out
 
 
+def emitter_wall(n_side: int, emission, z: float = -555.0, size: float = 555.0):
+    """The box's back wall as n_side x n_side light-only quads (two triangles
+    each, facing +z), quad k = (row k // n_side, column k % n_side) with
+    emission[k]."""
+    prims, lights = [], []
+    step = size / n_side
+    for k in range(n_side * n_side):
+        r, c = divmod(k, n_side)
+        x0, x1 = c * step, (c + 1) * step
+        y0, y1 = r * step, (r + 1) * step
+        prims.append(_quad((x0, y0, z), (x1, y0, z), (x1, y1, z), (x0, y1, z)))
+        lights.append({"type": "diffuse", "emission": [float(v) for v in emission[k]]})
+    return prims, lights
+
+
 def config3(width=512, height=512, spp=32, max_depth=8, freq=59, n_patches=4096, seed=2):
-    """Inverse-rendering scene: config 1's box and displaced sphere, with the
-    sphere cut into n_patches instances, each with its own GGX material
-    (albedo rgb, roughness) and its own diffuse light (emission rgb), so the
-    uniform light sampler covers all of them.  Parameters are the ground
-    truth drawn from `seed`; config3_params gives them and the start point."""
+    """Inverse-rendering scene on config 1's geometry: n_patches parameter
+    sets, each an RGB albedo, a roughness and an RGB emission.  The displaced
+    sphere is cut into n_patches GGX patch instances (albedo, roughness;
+    metallic drawn 0 or 1 per patch and held fixed), and the box's back wall
+    into n_patches light-only tiles (emission; sqrt(n_patches) a side), so the
+    uniform light sampler covers every tile.  Parameter set k is patch k's
+    material and tile k's light.  Emission sits on light-only instances
+    because the reference spawns rays without an origin offset: a surface
+    that both reflects and emits can hit itself at a distance that rounds to
+    zero, and the light pdf of that hit is 0/0 (a NaN radiance that covered
+    47% of the pixels when the patches carried their own emission).
+    Parameters are the ground truth drawn from `seed`; config3_params gives
+    them and the start point."""
+    n_side = int(round(n_patches ** 0.5))
+    if n_side * n_side != n_patches:
+        raise ValueError("n_patches must be a square")
     info = cornell_box(width, height, spp, max_depth)
     rng = np.random.default_rng(seed)
     alb = rng.uniform(0.2, 0.9, (n_patches, 3)).astype(np.float32)
     rough = rng.uniform(0.05, 0.8, n_patches).astype(np.float32)
+    metal = rng.integers(0, 2, n_patches).astype(np.float32)
     em = rng.uniform(0.0, 0.2, (n_patches, 3)).astype(np.float32)
+    # the back wall (primitive 2) gives way to the tiles
+    back = next(k for k, i in enumerate(info["instances"]) if i["primitive"] == 2)
+    info["nodes"][0]["instances"].remove(back)
     node = {"instances": [], "transform": len(info["transforms"])}
     info["transforms"].append({"keyframes": [{"time": 0.0, "scale": [90.0] * 3,
                                               "translation": [185.5, 300.0, -168.5]}]})
+    first_material = len(info["materials"])
     for k, prim in enumerate(split_mesh(displaced_sphere(freq, 0.08, 1), n_patches)):
         info["primitives"].append(prim)
-        info["materials"].append({"type": "pbr", "albedo": [float(x) for x in alb[k]], "metallic": 0.0,
+        info["materials"].append({"type": "pbr", "albedo": [float(x) for x in alb[k]], "metallic": float(metal[k]),
                                   "roughness": float(rough[k])})
-        info["lights"].append({"type": "diffuse", "emission": [float(x) for x in em[k]]})
         info["instances"].append({"primitive": len(info["primitives"]) - 1,
-                                  "material": len(info["materials"]) - 1, "light": len(info["lights"]) - 1})
+                                  "material": len(info["materials"]) - 1})
         node["instances"].append(len(info["instances"]) - 1)
     info["nodes"].append(node)
     info["nodes"][0]["children"].append(len(info["nodes"]) - 1)
-    info["x-patches"] = {"first_material": len(info["materials"]) - n_patches,
-                         "first_light": len(info["lights"]) - n_patches, "count": n_patches}
+    first_light = len(info["lights"])
+    prims, lights = emitter_wall(n_side, em)
+    for prim, light in zip(prims, lights):
+        info["primitives"].append(prim)
+        info["lights"].append(light)
+        info["instances"].append({"primitive": len(info["primitives"]) - 1, "light": len(info["lights"]) - 1})
+        info["nodes"][0]["instances"].append(len(info["instances"]) - 1)
+    info["x-patches"] = {"first_material": first_material, "first_light": first_light, "count": n_patches}
     return info
 
 
 def config3_params(sc, info):
     """(truth, start, mask): parameter arrays (material (n_mat, 4) albedo rgb +
-    roughness, light (n_light, 3) emission) of a loaded config-3 scene, the
-    start point (patch albedo 0.5, roughness 0.5, emission 0) and the mask
-    of the parameters being optimised (the patches)."""
+    roughness, light (n_light, 3) emission) of the scene as drawn, the
+    start point (patch albedo 0.5, roughness 0.5, tile emission 0) and the
+    mask of the parameters being optimised (patches and tiles)."""
     mat = np.array([[*m["albedo"], m["roughness"]] for m in sc.materials], np.float32)
     light = np.array([l["emission"] for l in sc.lights], np.float32)
     p = info["x-patches"]

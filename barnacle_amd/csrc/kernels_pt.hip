@@ -801,6 +801,13 @@ __device__ __forceinline__ void grad_accum(float* g, int key, float4 v, bool act
     }
 }
 __device__ __forceinline__ float ratio(float d, float f) { return f != 0.0f ? d / f : 0.0f; }
+// A gradient term that is not finite (0/0 or inf * 0 of a degenerate
+// configuration, e.g. a half vector perpendicular to wo in the pdf's
+// derivative) is dropped instead of added: one such term would turn the whole
+// parameter's gradient, and the optimiser state after it, into NaN.
+__device__ __forceinline__ void drop_nonfinite(int& key, const float4& v) {
+    if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w))) key = -1;
+}
 
 // k_shade's bounce, replayed: emission and its emission gradient; NEE set-up
 // with the terms its gradient needs (applied by k_trace_shadow_adj when the
@@ -918,6 +925,9 @@ __global__ __launch_bounds__(LDS ? kGradLdsThreads : 256) void k_shade_adj(
             P.L[slot] = f4(L, 0.0f);
             fin = !cont;
         }
+        drop_nonfinite(light_key, g_l);
+        drop_nonfinite(prev_key, g_p);
+        drop_nonfinite(mat_key, g_m);
         if (LDS) {
             lds_grad_light(lds_g, light_key, g_l, A.n_mat, A.n_light);
             lds_grad_mat(lds_g, prev_key, g_p, A.n_mat);
@@ -991,6 +1001,8 @@ __device__ __forceinline__ void nee_adj_item(const PathBuf& P, const AdjBuf& A, 
     }
     g_m = f4(gm, gr);
     P.L[slot] = f4(L, 0.0f);
+    drop_nonfinite(mat_key, g_m);
+    drop_nonfinite(light_key, g_l);
 }
 
 __global__ __launch_bounds__(256) void k_nee_adj(PathBuf P, AdjBuf A, const Ctl* ctl, const uint32_t* shq) {

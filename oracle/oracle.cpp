@@ -1504,13 +1504,26 @@ void liPTGrad(const orc_scene& s, Ray ray, Sampler& smp, int maxDepth, int rrDep
     Stats st;
     // one contribution c (rgb) reaching the film through beta: the earlier
     // BSDF factors' terms
+    // A term group (one parameter entry's rgb + roughness, or one light's
+    // rgb) that is not finite is dropped, as the GPU adjoint drops it.
+    auto add4 = [](double* g, const double v[4]) {
+        for (int k = 0; k < 4; ++k)
+            if (!std::isfinite(v[k])) return;
+        for (int k = 0; k < 4; ++k) g[k] += v[k];
+    };
+    auto add3 = [](double* g, const double v[3]) {
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(v[k])) return;
+        for (int k = 0; k < 3; ++k) g[k] += v[k];
+    };
     auto through = [&](const double c[3]) {
         for (const GradFactor& f : fac) {
-            double* g = gm + 4 * (size_t)f.mat;
+            double v[4] = {0, 0, 0, 0};
             for (int ch = 0; ch < 3; ++ch) {
-                g[ch] += delta[ch] * c[ch] * f.la[ch];
-                g[3] += delta[ch] * c[ch] * f.lr[ch];
+                v[ch] = delta[ch] * c[ch] * f.la[ch];
+                v[3] += delta[ch] * c[ch] * f.lr[ch];
             }
+            add4(gm + 4 * (size_t)f.mat, v);
         }
     };
     while (depth < maxDepth) {
@@ -1528,13 +1541,17 @@ void liPTGrad(const orc_scene& s, Ray ray, Sampler& smp, int maxDepth, int rrDep
             const LightD& ld = s.lightDefs[in.light];
             double facing = std::fabs(woz) > 1e-6f && (woz > 0.0f || ld.twoSided) ? 1.0 : 0.0;
             double bw[3] = {beta.x, beta.y, beta.z};
-            for (int ch = 0; ch < 3; ++ch) gl[3 * (size_t)in.light + ch] += delta[ch] * bw[ch] * w * facing;
+            double ve[3];
+            for (int ch = 0; ch < 3; ++ch) ve[ch] = delta[ch] * bw[ch] * w * facing;
+            add3(gl + 3 * (size_t)in.light, ve);
             if (depth > 0 && prevMat >= 0) {
                 // w = prev / (lpdf + prev): d w / d prev = lpdf / (lpdf + prev)^2
                 double sden = (double)le.pdf + prevPdf;
                 double dw = (double)le.pdf / (sden * sden) * dPrevPdf;
                 double lv[3] = {le.L.x, le.L.y, le.L.z};
-                for (int ch = 0; ch < 3; ++ch) gm[4 * (size_t)prevMat + 3] += delta[ch] * bw[ch] * lv[ch] * dw;
+                double vp[4] = {0, 0, 0, 0};
+                for (int ch = 0; ch < 3; ++ch) vp[3] += delta[ch] * bw[ch] * lv[ch] * dw;
+                add4(gm + 4 * (size_t)prevMat, vp);
             }
         }
         if (in.material < 0) break;
@@ -1554,18 +1571,20 @@ void liPTGrad(const orc_scene& s, Ray ray, Sampler& smp, int maxDepth, int rrDep
             through(c);
             double da[3], dr[3], dpb;
             matGradD(m, toLocal(h.g.onb, -ray.d), toLocal(h.g.onb, ls.wi), e.pdf, da, dr, &dpb);
-            double* g = gm + 4 * (size_t)in.material;
             double bl[3] = {(double)beta.x * lb.x, (double)beta.y * lb.y, (double)beta.z * lb.z};
             // d / d roughness of f / (pdf_b + pdf_l): f' / S - f * pdf_b' / S^2
             double dmis = dpb * (double)rc;
+            double vm[4] = {0, 0, 0, 0};
             for (int ch = 0; ch < 3; ++ch) {
-                g[ch] += delta[ch] * bl[ch] * da[ch];
-                g[3] += delta[ch] * bl[ch] * dr[ch] - delta[ch] * c[ch] * dmis;
+                vm[ch] = delta[ch] * bl[ch] * da[ch];
+                vm[3] += delta[ch] * bl[ch] * dr[ch] - delta[ch] * c[ch] * dmis;
             }
+            add4(gm + 4 * (size_t)in.material, vm);
             // the sampled light's emission (Light.fs:49-52 facing test)
             const double av[3] = {a.x, a.y, a.z};
-            for (int ch = 0; ch < 3; ++ch)
-                gl[3 * (size_t)ls.ldef + ch] += delta[ch] * av[ch] * rc * ls.facing;
+            double vl[3];
+            for (int ch = 0; ch < 3; ++ch) vl[ch] = delta[ch] * av[ch] * rc * ls.facing;
+            add3(gl + 3 * (size_t)ls.ldef, vl);
         }
         float ul = smp.next();
         V2 ub = smp.next2();

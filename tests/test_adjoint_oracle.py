@@ -42,7 +42,7 @@ def test_pbr_derivatives_match_central_differences(orc):
         assert np.allclose(f32, f, rtol=1e-4, atol=1e-6)
 
 
-def small_scene(max_depth, rr_depth, spp=4, res=20, patches=12):
+def small_scene(max_depth, rr_depth, spp=4, res=20, patches=16):
     info = configs.config3(width=res, height=res, spp=spp, max_depth=max_depth, freq=6, n_patches=patches)
     info["integrator"]["rr-depth"] = rr_depth
     sc = scene.load_scene(info)
@@ -96,7 +96,7 @@ def test_roughness_gradient_through_nee_matches_fd(orc):
     fm = info["x-patches"]["first_material"]
     # a wider step: the loss is smooth in roughness here, and float32 films
     # leave ~1e-5 of noise in a 1e-3 difference quotient
-    res = fd_check(orc, sc, info, "mat", [(fm + k, 3) for k in range(0, 12, 3)], h=3e-3)
+    res = fd_check(orc, sc, info, "mat", [(fm + k, 3) for k in range(16)], h=3e-3)
     assert any(abs(fd) > 1e-3 for fd, _ in res)
     for fd, ad in res:
         assert abs(fd - ad) <= 0.01 * abs(fd) + 1e-5, (fd, ad)

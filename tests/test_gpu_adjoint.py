@@ -10,7 +10,7 @@ from barnacle_amd import configs, scene
 pytestmark = pytest.mark.gpu
 
 
-def make(max_depth=6, rr_depth=3, spp=4, res=24, patches=12, freq=6):
+def make(max_depth=6, rr_depth=3, spp=4, res=24, patches=16, freq=6):
     info = configs.config3(width=res, height=res, spp=spp, max_depth=max_depth, freq=freq, n_patches=patches)
     info["integrator"]["rr-depth"] = rr_depth
     return info, scene.load_scene(info)
