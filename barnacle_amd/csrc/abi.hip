@@ -165,14 +165,19 @@ std::vector<float4> pack_materials(const std::vector<bn_material>& mats) {
     std::vector<float4> d(2 * std::max<size_t>(1, mats.size()));
     for (size_t i = 0; i < mats.size(); ++i) {
         const bn_material& m = mats[i];
-        int kind = m.kind;
+        const bool given_alpha = m.kind == BN_MAT_PBR_ALPHA;
+        int kind = given_alpha ? BN_MAT_PBR : m.kind;
         float kf;
         std::memcpy(&kf, &kind, 4);
         float r2 = m.roughness * m.roughness;
         float alpha = (r2 != r2) ? r2 : (r2 > 1e-3f ? r2 : 1e-3f);
+        // BN_MAT_PBR_ALPHA: the field holds PBRMaterial.Alpha itself; the
+        // roughness kept for d alpha / d roughness is its square root
+        const float rough = given_alpha ? std::sqrt(m.roughness) : m.roughness;
+        if (given_alpha) alpha = m.roughness;
         float met = m.metallic < 0.0f ? 0.0f : (m.metallic > 1.0f ? 1.0f : m.metallic);
         d[2 * i] = make_float4(kf, m.albedo[0], m.albedo[1], m.albedo[2]);
-        d[2 * i + 1] = make_float4(m.ior, alpha, met, m.roughness);
+        d[2 * i + 1] = make_float4(m.ior, alpha, met, rough);
     }
     return d;
 }
@@ -243,7 +248,7 @@ void upload_scene(Ctx* c, SceneRec& sc, const std::vector<bn_bvh_node>& tlas,
         if (in.light >= 0) lrec.push_back(make_int4((int)i, d.info.y, ntri, in.light));
     }
     for (const auto& m : mats)
-        if (m.kind < 0 || m.kind > 3) throw Error(BN_E_INVALID, "unknown material kind");
+        if (m.kind < 0 || m.kind > BN_MAT_PBR_ALPHA) throw Error(BN_E_INVALID, "unknown material kind");
     sc.mats_host = mats;
     sc.lights_host = lights;
     const std::vector<float4> dmats = pack_materials(mats);

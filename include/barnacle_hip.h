@@ -61,6 +61,11 @@ typedef struct bn_primitive {          /* PrimitiveInfo (Loader.fs:92-116)   */
 #define BN_MAT_MIRROR      1
 #define BN_MAT_DIELECTRIC  2
 #define BN_MAT_PBR         3
+/* PBRMaterial handed over as built: `roughness` holds PBRMaterial.Alpha
+ * (= Max(roughness^2, 1e-3), PBR.fs:11) itself, which is what a host that
+ * holds the constructed material object has; metallic is clamped as for
+ * BN_MAT_PBR.  Upload path only (bn_upload_scene). */
+#define BN_MAT_PBR_ALPHA   4
 
 typedef struct bn_material {           /* MaterialInfo (Loader.fs:50-73)     */
     int32_t kind;                      /* BN_MAT_*                           */

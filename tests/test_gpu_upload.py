@@ -145,3 +145,28 @@ def test_scene_release_frees_its_blas_and_keeps_others(ctx):
     assert np.array_equal(fa, b.render_pt(p))
     c = ctx.scene(sc)
     assert np.array_equal(fa, c.render_pt(p))
+
+
+def test_pbr_given_alpha_matches_pbr_given_roughness(ctx):
+    """A host holding constructed PBRMaterial objects has Alpha =
+    Max(roughness^2, 1e-3) (PBR.fs:11), not roughness: uploaded as
+    BN_MAT_PBR_ALPHA with that alpha, the scene renders bit-identically to
+    the BN_MAT_PBR scene of the same roughness."""
+    sc = scene.load_scene(CASES["config1"]())
+    built = ctx.scene(sc)
+    p = sc.pt_params()
+    ref = built.render_pt(p)
+    mats = sc.materials
+    alt = [dict(m) for m in mats]
+    for m in alt:
+        if m["kind"] == 3:
+            r = np.float32(m["roughness"])
+            m["kind"] = 4
+            m["roughness"] = max(np.float32(r * r), np.float32(1e-3))
+    sc.materials = alt
+    try:
+        up, _ = reupload(ctx, sc, built)
+    finally:
+        sc.materials = mats
+    assert any(m["kind"] == 4 for m in alt)
+    assert np.array_equal(up.render_pt(p).view(np.int32), ref.view(np.int32))
