@@ -126,8 +126,10 @@ def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
     return out
 
 
-def batched(ctx, n=32):
-    """Plain per-scene renders against one batched render of the same items."""
+def batched(ctx, n=32, pause_s=0.0, small=True):
+    """Plain per-scene renders against one batched render of the same items.
+    pause_s > 0 idles the GPU that long between the warm-up and timed phases,
+    so that a kernel trace of the run splits into them at its longest gaps."""
     t0 = time.perf_counter()
     scs = [scene.load_scene(j) for j in configs.config2(n)]
     ds = [ctx.scene(s) for s in scs]
@@ -135,20 +137,69 @@ def batched(ctx, n=32):
     setup = time.perf_counter() - t0
     for d, p in jobs:
         d.render_pt(p)
+    time.sleep(pause_s)
     t0 = time.perf_counter()
     plain = [d.render_pt(p) for d, p in jobs]
     plain_s = time.perf_counter() - t0
+    time.sleep(pause_s)
     ctx.render_batch(jobs)
+    time.sleep(pause_s)
     t0 = time.perf_counter()
     films, st = ctx.render_batch(jobs)
     batch_s = time.perf_counter() - t0
+    time.sleep(pause_s)
     equal = all(np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(films, plain))
     paths = sum(p.width * p.height * p.spp for _, p in jobs)
     return {"config": f"config2: {n} seeded scenes, 1-256 instances of 1k-500k-triangle displaced icospheres, "
                       "256^2-1024^2, 4-64 spp, depth 4-16",
             "paths": paths, "plain_s": plain_s, "batched_s": batch_s, "speedup": plain_s / batch_s,
             "batched_mpaths_per_s": paths / batch_s / 1e6, "rounds": st["rounds"], "graph": st["graph"],
-            "films_bit_equal": equal, "scene_setup_s": setup, "small_items": small_items(ctx)}
+            "films_bit_equal": equal, "scene_setup_s": setup, "small_items": small_items(ctx) if small else None}
+
+
+def timeline_summary(csv_path, min_phase_gap_s=0.2):
+    """Idle time of the GPU in a rocprofv3 kernel trace (kernel_trace.csv) of
+    `bench.py --batch-timeline`: the trace splits into phases at gaps of at
+    least min_phase_gap_s (warm-up plain, timed plain, warm-up batch, timed
+    batch).  Per phase: span, time with no kernel running, the gaps between
+    consecutive kernels by size, and the time per kernel."""
+    import csv
+    with open(csv_path) as fh:
+        rows = list(csv.DictReader(fh))
+    ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in rows)
+    phases, cur = [], [ks[0]]
+    for k in ks[1:]:
+        if k[0] - max(e for _, e, _ in cur) >= min_phase_gap_s * 1e9:
+            phases.append(cur)
+            cur = []
+        cur.append(k)
+    phases.append(cur)
+    names = ["plain (warm-up)", "plain (timed)", "batch (warm-up)", "batch (timed)"]
+    names = ["setup"] * max(0, len(phases) - 4) + names[-len(phases):]   # uploads come first
+    out = []
+    for i, ph in enumerate(phases):
+        span = ph[-1][1] - ph[0][0]
+        busy, end, gaps = 0, ph[0][0], []
+        for s0, e0, _ in ph:                                  # union of kernel intervals
+            if s0 > end:
+                gaps.append(s0 - end)
+            busy += max(0, e0 - max(s0, end))
+            end = max(end, e0)
+        bins = [(0, 10e3), (10e3, 100e3), (100e3, 1e6), (1e6, float("inf"))]
+        hist = {f"{int(lo / 1e3)}-{'inf' if hi == float('inf') else int(hi / 1e3)}us":
+                {"count": sum(lo <= g < hi for g in gaps), "ms": sum(g for g in gaps if lo <= g < hi) / 1e6}
+                for lo, hi in bins}
+        per = {}
+        for s0, e0, nm in ph:
+            short = nm.split("(")[0].split("<")[0].replace("void ", "").strip()
+            c, t = per.get(short, (0, 0.0))
+            per[short] = (c + 1, t + (e0 - s0) / 1e6)
+        out.append({"phase": names[i], "kernels": len(ph),
+                    "span_ms": span / 1e6, "busy_ms": busy / 1e6, "idle_ms": (span - busy) / 1e6,
+                    "idle_frac": (span - busy) / span, "gaps": hist,
+                    "per_kernel_ms": {k: round(v[1], 3) for k, v in sorted(per.items(), key=lambda kv: -kv[1][1])},
+                    "per_kernel_launches": {k: v[0] for k, v in per.items()}})
+    return out
 
 
 def small_items(ctx, n=256):
@@ -352,10 +403,23 @@ def main():
     ap.add_argument("--no-pssmlt", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--batch-timeline", action="store_true",
+                    help="run only the config-2 plain/batched leg, with pauses between its phases")
+    ap.add_argument("--timeline-summary", metavar="KERNEL_TRACE_CSV",
+                    help="summarise a rocprofv3 kernel trace of --batch-timeline (no GPU use)")
     args = ap.parse_args()
+    if args.timeline_summary:
+        print(json.dumps(timeline_summary(args.timeline_summary), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the renderer has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_timeline:
+        ctx = native.Context(0)
+        r = batched(ctx, pause_s=0.5, small=False)
+        print(json.dumps({k: v for k, v in r.items() if k != "small_items"}), flush=True)
+        return
 
     sc = scene.load_scene(configs.config1(spp=args.spp))
     params = sc.pt_params()

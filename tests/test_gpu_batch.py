@@ -146,3 +146,27 @@ def test_batch_ragged_and_empty_items(ctx):
             ctx.set_option("graph", 1)
         for (d, p), f in zip(jobs, films):
             assert np.array_equal(f.view(np.int32), d.render_pt(p).view(np.int32))
+
+
+def test_all_32_generator_scenes_batch_equals_plain(ctx, orc):
+    """The 32-scene batch at its real geometry (BLAS of 1k-500k triangles,
+    1-256 instances per scene) with films cut to at most 96 x 96 and 4 spp:
+    the batch of all 32 seeds equals the 32 plain renders bit for bit, and
+    the scene with the most triangles has first hits bit-exact against the
+    oracle."""
+    infos = configs.config2(32, max_res=96, max_spp=4)
+    scs = [scene.load_scene(i) for i in infos]
+    print(f"32 generator scenes: {sum(s.n_triangles for s in scs)} triangles", flush=True)
+    jobs = [(ctx.scene(s), s.pt_params()) for s in scs]
+    films, st = ctx.render_batch(jobs)
+    assert st["rounds"] >= 1
+    for (ds, p), f in zip(jobs, films):
+        assert np.array_equal(f.view(np.int32), ds.render_pt(p).view(np.int32))
+    assert sum(np.isfinite(f).all(1).mean() > 0.5 for f in films) >= 28
+    k = int(np.argmax([s.n_triangles for s in scs]))
+    assert max(pr["n_indices"] for pr in scs[k].primitives if pr.get("n_indices")) // 3 > 100000
+    p = scs[k].pt_params()
+    gi, gp, gt = jobs[k][0].first_hit(p, 0)
+    oi, op, ot = orc.scene(scs[k]).first_hit(p, 0)
+    assert np.array_equal(gi, oi) and np.array_equal(gp, op)
+    assert np.array_equal(gt.view(np.int32), ot.view(np.int32))
