@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libbarnacle_hip.so")
-SOURCES = ["abi.hip", "kernels_pt.hip", "kernels_mlt.hip", "noalloc.hip", "host_scene.cpp", "film.cpp"]
+SOURCES = ["abi.hip", "kernels_pt.hip", "kernels_mlt.hip", "noalloc.hip", "bvh_gpu.hip", "host_scene.cpp",
+           "film.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -18,6 +18,7 @@
 
 #include "../../include/barnacle_hip.h"
 #include "abi_internal.hpp"
+#include "bvh_gpu.hpp"
 #include "host_scene.hpp"
 #include "kernels.hpp"
 #include "kernels_mlt.hpp"
@@ -1112,6 +1113,22 @@ int bn_bvh_build(const float* b, int32_t n, bn_bvh_node* out_nodes, int32_t* out
     });
 }
 
+int bn_bvh_build_gpu(bn_ctx* ctx, const float* b, int32_t n, bn_bvh_node* out_nodes, int32_t* out_n_nodes,
+                     int32_t* out_perm) {
+    return guard([&] {
+        Ctx* c = reinterpret_cast<Ctx*>(ctx);
+        if (!c || n <= 0 || !b || !out_nodes || !out_n_nodes || !out_perm) throw Error(BN_E_INVALID, "bad arguments");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        std::vector<std::vector<bn_bvh_node>> nodes;
+        std::vector<std::vector<int32_t>> perm;
+        bvh_build_gpu(c, {b}, {n}, nodes, perm);
+        if ((int)nodes[0].size() > *out_n_nodes) throw Error(BN_E_CAPACITY, "node buffer too small");
+        std::memcpy(out_nodes, nodes[0].data(), nodes[0].size() * sizeof(bn_bvh_node));
+        *out_n_nodes = (int)nodes[0].size();
+        std::memcpy(out_perm, perm[0].data(), perm[0].size() * 4);
+    });
+}
+
 int bn_upload_blas(bn_ctx* ctx, const bn_bvh_node* nodes, int32_t n_nodes, const float* verts, int32_t n_verts,
                    const int32_t* tri_idx, int32_t n_tris, const float* alias_pdf, int32_t* out_blas) {
     return guard([&] {
@@ -1188,7 +1205,12 @@ int bn_scene_build(bn_ctx* ctx, const bn_scene_desc* desc, int32_t* out_scene) {
         if (!c || !desc || !out_scene) throw Error(BN_E_INVALID, "bad arguments");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto hs = std::make_shared<bnh::HostScene>();
-        bnh::build_scene(*desc, *hs);
+        const bnh::BatchBvhBuild gpu_build = [c](const std::vector<const float*>& sets, const std::vector<int>& counts,
+                                                 std::vector<std::vector<bn_bvh_node>>& nodes,
+                                                 std::vector<std::vector<int32_t>>& perms) {
+            bvh_build_gpu(c, sets, counts, nodes, perms);
+        };
+        bnh::build_scene(*desc, *hs, c->bvh_gpu ? &gpu_build : nullptr);
         std::vector<int> blas_of(hs->meshes.size(), -1);
         for (size_t i = 0; i < hs->meshes.size(); ++i) {
             const bnh::HostMesh& m = hs->meshes[i];
@@ -1431,6 +1453,9 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "batch_pack") {
             if (value < 0 || value > 1) throw Error(BN_E_INVALID, "batch_pack must be 0 or 1");
             c->batch_pack = (int)value;
+        } else if (k == "bvh_gpu") {
+            if (value < 0 || value > 1) throw Error(BN_E_INVALID, "bvh_gpu must be 0 or 1");
+            c->bvh_gpu = value != 0;
         } else if (k == "refill") {
             if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
             c->refill = (int)value;

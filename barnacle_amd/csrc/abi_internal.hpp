@@ -113,6 +113,10 @@ struct Ctx {
     size_t mlt_chunk = (size_t)1 << 20;   // chains (or bootstrap paths) in flight
     DevBuf m_inner, m_outer, m_state, m_bid, m_pix_new, m_pix_cur, m_Lcur, m_rad, m_acc, m_xs;
     DevBuf m_weights, m_aprob, m_aidx, m_accepted_out;
+    // GPU BVH builder (bvh_gpu.hip): items (two buffers, then the output
+    // nodes), node regions, permutation, task lists, chunk partials
+    bool bvh_gpu = true;                   // bn_scene_build builds finite meshes' BLAS on the GPU
+    DevBuf g_items, g_region, g_perm, g_rsize, g_big, g_info, g_small, g_chunks, g_part, g_unodes, g_misc;
     std::vector<float> last_bootstrap;     // of the last bn_render_pssmlt
     std::vector<int32_t> last_chain_accepts;
     ~Ctx() {

@@ -389,12 +389,16 @@ AliasTable build_alias(const float* w, int n) {
 }
 
 // ---------------------------------------------------------------- scene
-void build_scene(const bn_scene_desc& d, HostScene& S) {
+void build_scene(const bn_scene_desc& d, HostScene& S, const BatchBvhBuild* blas_build) {
+    static_assert(sizeof(Box) == 6 * sizeof(float), "Box is six packed floats");
     S = HostScene{};
     S.camera = d.camera;
     for (int i = 0; i < d.n_materials; ++i) S.materials.push_back(d.materials[i]);
     for (int i = 0; i < d.n_lights; ++i) S.light_defs.push_back(d.lights[i]);
     S.meshes.resize(d.n_primitives);
+    std::vector<std::vector<Box>> tri_boxes(d.n_primitives);
+    std::vector<std::vector<int32_t>> perms(d.n_primitives);
+    std::vector<int> deferred;                                        // meshes for blas_build
     for (int pi = 0; pi < d.n_primitives; ++pi) {
         const bn_primitive& p = d.primitives[pi];
         HostMesh& m = S.meshes[pi];
@@ -411,15 +415,43 @@ void build_scene(const bn_scene_desc& d, HostScene& S) {
         }
         const int32_t* ix = d.indices + p.index_offset;
         const int nt = p.n_indices / 3;
-        std::vector<Box> tb(nt);
+        std::vector<Box>& tb = tri_boxes[pi];
+        tb.resize(nt);
+        bool finite = true;
         for (int k = 0; k < nt; ++k) {
             for (int j = 0; j < 3; ++j)
                 if (ix[3 * k + j] < 0 || ix[3 * k + j] >= p.n_verts) throw Error(BN_E_INVALID, "mesh index out of range");
             V3 a = m.verts[ix[3 * k]], b = m.verts[ix[3 * k + 1]], c = m.verts[ix[3 * k + 2]];
             tb[k] = {vmin(vmin(a, b), c), vmax(vmax(a, b), c)};        // Mesh.fs:162-168
+            finite = finite && std::isfinite(tb[k].lo.x) && std::isfinite(tb[k].lo.y) && std::isfinite(tb[k].lo.z) &&
+                     std::isfinite(tb[k].hi.x) && std::isfinite(tb[k].hi.y) && std::isfinite(tb[k].hi.z);
         }
-        std::vector<int32_t> perm;
-        build_bvh(tb, m.nodes, perm);
+        if (blas_build && finite) deferred.push_back(pi);
+    }
+    if (!deferred.empty()) {
+        std::vector<const float*> sets;
+        std::vector<int> counts;
+        for (int pi : deferred) {
+            sets.push_back(reinterpret_cast<const float*>(tri_boxes[pi].data()));
+            counts.push_back((int)tri_boxes[pi].size());
+        }
+        std::vector<std::vector<bn_bvh_node>> nodes;
+        std::vector<std::vector<int32_t>> built;
+        (*blas_build)(sets, counts, nodes, built);
+        for (size_t k = 0; k < deferred.size(); ++k) {
+            S.meshes[deferred[k]].nodes = std::move(nodes[k]);
+            perms[deferred[k]] = std::move(built[k]);
+        }
+    }
+    for (int pi = 0; pi < d.n_primitives; ++pi) {
+        const bn_primitive& p = d.primitives[pi];
+        HostMesh& m = S.meshes[pi];
+        if (m.sphere) continue;
+        const int32_t* ix = d.indices + p.index_offset;
+        const int nt = p.n_indices / 3;
+        if (perms[pi].empty()) build_bvh(tri_boxes[pi], m.nodes, perms[pi]);
+        std::vector<Box>().swap(tri_boxes[pi]);
+        const std::vector<int32_t>& perm = perms[pi];
         m.tri.resize(3 * (size_t)nt);
         for (int k = 0; k < nt; ++k)
             for (int j = 0; j < 3; ++j) m.tri[3 * k + j] = ix[3 * perm[k] + j];

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -69,7 +70,7 @@ M4 keyframe_matrix(const bn_transform& t);       // KeyFrameInfo.ToKeyFrame
 M4 transform_eval(const bn_transform* keys, int n, float time);   // Transform.Eval
 
 struct Box {
-    V3 lo, hi;
+    V3 lo, hi;                                    // 6 contiguous floats
     static Box empty() { return {{kInf, kInf, kInf}, {-kInf, -kInf, -kInf}}; }
     void grow(const Box& b) { lo = vmin(lo, b.lo); hi = vmax(hi, b.hi); }
     void grow(V3 p) { lo = vmin(lo, p); hi = vmax(hi, p); }
@@ -123,8 +124,16 @@ struct HostScene {
     M4 c2w{};
 };
 
+// Builds the BVHs of several item sets at once (sets[s]: counts[s] boxes of 6
+// floats, all finite) with the nodes and permutations of build_bvh.
+using BatchBvhBuild = std::function<void(const std::vector<const float*>& sets, const std::vector<int>& counts,
+                                         std::vector<std::vector<bn_bvh_node>>& nodes,
+                                         std::vector<std::vector<int32_t>>& perms)>;
+
 // Scene.Load + Traverse + BVHAggregate + UniformLightSampler construction.
-void build_scene(const bn_scene_desc& d, HostScene& out);
+// With blas_build, the BLAS of every mesh whose triangle bounds are finite
+// are built by it in one call; the others, and the TLAS, by build_bvh.
+void build_scene(const bn_scene_desc& d, HostScene& out, const BatchBvhBuild* blas_build = nullptr);
 
 // Film.PostProcess with tone mapping 0 identity, 1 ACES, 2 gamma, clamped to
 // [0, 1], and the 8-bit RGBA of Film.Save (film.cpp).  Either output may be

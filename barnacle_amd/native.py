@@ -36,6 +36,7 @@ _SIGS = {
     "bn_abi_version": ([], C.c_int),
     "bn_struct_sizes": ([i32p, C.c_int32], C.c_int),
     "bn_bvh_build": ([f32p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p], C.c_int),
+    "bn_bvh_build_gpu": ([C.c_void_p, f32p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p], C.c_int),
     "bn_scene_build": ([C.c_void_p, C.POINTER(abi.SceneDesc), i32p], C.c_int),
     "bn_scene_release": ([C.c_void_p, C.c_int32], C.c_int),
     "bn_scene_export": ([C.c_void_p, C.c_int32, C.POINTER(abi.BVHNode), i32p, i32p, i32p, f32p, f32p,
@@ -131,6 +132,17 @@ class Context:
 
     def set_option(self, key: str, value: int):
         check(self.lib.bn_set_option(self.h, key.encode(), int(value)))
+
+    def bvh_build(self, bounds):
+        """BVHNode.Build (Util/BVH.fs:128-247) on this context's GPU: (nodes, perm)."""
+        b = np.ascontiguousarray(bounds, np.float32).reshape(-1, 6)
+        n = len(b)
+        nodes = (abi.BVHNode * (2 * n + 1))()
+        nn = C.c_int32(2 * n + 1)
+        perm = np.zeros(n, np.int32)
+        check(self.lib.bn_bvh_build_gpu(self.h, _p(b, f32p), n, nodes, C.byref(nn), _p(perm, i32p)))
+        from .bvh_util import nodes_to_numpy
+        return nodes_to_numpy(nodes, nn.value), perm
 
     def kernel_times(self):
         """{kernel name: (launches, total ms)} accumulated since the last reset."""

@@ -212,6 +212,14 @@ int         bn_struct_sizes(int32_t* out, int32_t n);
 int bn_bvh_build(const float* item_bounds, int32_t n_items,
                  bn_bvh_node* out_nodes, int32_t* out_n_nodes, int32_t* out_perm);
 
+/* BVHNode.Build (Util/BVH.fs:128-247) on the context's GPU: the same nodes,
+ * node order and permutation as bn_bvh_build, bit for bit, for finite item
+ * bounds (BN_E_INVALID otherwise; bn_bvh_build takes any).  bn_scene_build
+ * builds the BLAS of every mesh with finite triangle bounds this way unless
+ * the option "bvh_gpu" is 0. */
+int bn_bvh_build_gpu(bn_ctx* ctx, const float* item_bounds, int32_t n_items,
+                     bn_bvh_node* out_nodes, int32_t* out_n_nodes, int32_t* out_perm);
+
 /* Scene construction as Scene.Load + Scene.Render(0) perform it before the
  * timed region (Loader.fs:253-272, Base/Scene.fs:38-51, Render.fs:11-14):
  * every mesh BLAS is built (Mesh.fs:159-169), the node graph is traversed DFS
@@ -362,7 +370,9 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "material_sort"        1: bin the shading queue by material kind; 0: queue
  *                          order, which keeps path-state access coalesced;
  *                          -1 (default): bins for gradient rounds only, whose
- *                          atomics gain from material-coherent waves */
+ *                          atomics gain from material-coherent waves
+ *   "bvh_gpu"              1 (default): bn_scene_build builds mesh BLAS on the
+ *                          GPU (bn_bvh_build_gpu); 0: on the host */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {
