@@ -1456,6 +1456,9 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "bvh_gpu") {
             if (value < 0 || value > 1) throw Error(BN_E_INVALID, "bvh_gpu must be 0 or 1");
             c->bvh_gpu = value != 0;
+        } else if (k == "bvh_small") {
+            if (value < 128 || value > 512) throw Error(BN_E_INVALID, "bvh_small must be in [128, 512]");
+            c->bvh_small = (int)value;
         } else if (k == "refill") {
             if (value < 1 || value > 64) throw Error(BN_E_INVALID, "refill must be in [1, 64]");
             c->refill = (int)value;

@@ -116,10 +116,14 @@ struct Ctx {
     // GPU BVH builder (bvh_gpu.hip): items (two buffers, then the output
     // nodes), node regions, permutation, task lists, chunk partials
     bool bvh_gpu = true;                   // bn_scene_build builds finite meshes' BLAS on the GPU
+    int bvh_small = 512;                   // largest task one wave builds whole (at most 512)
     DevBuf g_items, g_region, g_perm, g_rsize, g_big, g_info, g_small, g_chunks, g_part, g_unodes, g_misc;
+    void* g_pinned = nullptr;              // page-locked staging (hipHostMalloc), grown on demand
+    size_t g_pinned_bytes = 0;
     std::vector<float> last_bootstrap;     // of the last bn_render_pssmlt
     std::vector<int32_t> last_chain_accepts;
     ~Ctx() {
+        if (g_pinned) (void)hipHostFree(g_pinned);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

@@ -14,13 +14,13 @@
 //    order from `last - 1` downwards (Util/BVH.fs:205-217).
 //  * The SAH sweep of a node runs in one thread in the reference's order.
 //
-// Schedule: a task is a node's item range.  Tasks of more than kSmall items
+// Schedule: a task is a node's item range.  Tasks of more than `small` items
 // run level by level over chunks of kChunk items (one block per chunk):
 // bounds, bins and the scatter are chunk kernels, the per-node decisions one
-// thread per task.  A task of at most kSmall items (or at depth 64) is built
+// thread per task.  A task of at most `small` items (or at depth 64) is built
 // whole by one wave, recursively with an LDS stack, into a node region of its
-// own.  A final pass numbers the upper nodes and regions in pre-order and
-// writes the flattened nodes.
+// own.  The host numbers the upper nodes and regions in pre-order, and a
+// last pass writes the flattened nodes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,7 +36,6 @@ namespace bvhg {
 constexpr int kBins = 12, kLeaf = 4, kMaxDepth = 64;        // BVHBuildConfig (Util/BVH.fs:99-107)
 constexpr int kThreads = 256, kIpt = 8;
 constexpr int kChunk = kThreads * kIpt;                     // items per chunk block
-constexpr int kSmall = kChunk;                              // largest task one wave builds
 constexpr int kWavesPerBlock = kThreads / 64;
 constexpr float kInf = __builtin_huge_valf();
 
@@ -46,7 +45,7 @@ struct DBox { float l[3], h[3]; };
 struct BigTask { int first, last, depth, unode, set, chunk0, nchunks, pad; };
 struct BigInfo { int mode, axis, best, nl; float lo_axis, extent, pad0, pad1; DBox cb; };   // mode 0 bin, 1 middle
 struct SmallTask { int first, last, depth, buf, set, pad0, pad1, pad2; };
-struct UNode { DBox b; int axis, left, right, set; };      // child < 0: region ~first
+struct UNode { DBox b; int axis, left, right, set; };      // child < 0: small task ~child
 struct Ctl { int n_next_big, n_small, n_unodes, n_chunks, err, pad[3]; };
 
 __device__ __forceinline__ float omin(float a, float b) { return a < b ? a : b; }
@@ -180,7 +179,7 @@ struct Bufs {
     int* chunk_rbase;
     UNode* unodes;
     bn_bvh_node* region; // 2 * first: the region of the small task starting at `first`
-    int* region_size;    // at `first`
+    int* small_size;     // nodes of each small task's region
     int32_t* perm;       // per position: item id
     const int* set_off;
     Ctl* ctl;
@@ -271,7 +270,7 @@ __global__ void __launch_bounds__(kThreads) k_chunk_bins(Bufs B, int src) {
 
 // K4: per task: SAH sweep (Util/BVH.fs:183-203), chunk scatter bases and the
 // two children, each either a task of the next level or a small task.
-__global__ void k_task_split(Bufs B, int n_big, int level) {
+__global__ void k_task_split(Bufs B, int n_big, int level, int small) {
     const int ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= n_big) return;
     const BigTask t = B.big[ti];
@@ -331,7 +330,7 @@ __global__ void k_task_split(Bufs B, int n_big, int level) {
     int code[2];
     for (int side = 0; side < 2; ++side) {
         const int f = ranges[side][0], l = ranges[side][1];
-        if (l - f > kSmall && t.depth + 1 < kMaxDepth) {
+        if (l - f > small && t.depth + 1 < kMaxDepth) {
             const int u = atomicAdd(&B.ctl->n_unodes, 1);
             const int k = atomicAdd(&B.ctl->n_next_big, 1);
             if (u >= B.unode_cap || k >= B.big_cap) { atomicOr(&B.ctl->err, 2); code[side] = -1; continue; }
@@ -341,7 +340,7 @@ __global__ void k_task_split(Bufs B, int n_big, int level) {
             const int k = atomicAdd(&B.ctl->n_small, 1);
             if (k >= B.small_cap) { atomicOr(&B.ctl->err, 2); code[side] = -1; continue; }
             B.small[k] = {f, l, t.depth + 1, (level + 1) & 1, t.set, 0, 0, 0};
-            code[side] = ~f;
+            code[side] = ~k;
         }
     }
     B.unodes[t.unode].left = code[0];
@@ -438,6 +437,10 @@ __device__ __forceinline__ bn_bvh_node make_node(const DBox& b, int32_t word, bo
 
 // One wave builds the whole subtree of a small task (Tree.Build, recursive,
 // left first) into the task's node region, numbered in pre-order from 0.
+// A task has at most 64 * kPer items, so each lane holds its run of at most
+// kPer consecutive items in registers while a node is processed.
+constexpr int kPer = 8;
+constexpr int kSmallMax = 64 * kPer;
 __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
     struct Frame { int first, last, depth, parent, buf; };
     __shared__ Frame stack[kWavesPerBlock][kMaxDepth + 2];
@@ -455,15 +458,18 @@ __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
         const int me = nodes++;
         if (f.parent >= 0 && lane == 0) region[f.parent].right_or_offset = me;
         const int cnt = f.last - f.first;
-        const int per = (cnt + 63) / 64;
-        const int b = min(f.first + lane * per, f.last), en = min(b + per, f.last);
+        const int per = (cnt + 63) / 64;                  // at most kPer
+        const int b = min(f.first + lane * per, f.last), n = min(b + per, f.last) - b;
         const Item* in = B.items[f.buf];
+        Item its[kPer];
         DBox bb = empty_box(), cb = empty_box();
-        for (int i = b; i < en; ++i) {
-            const Item it = load_item(in + i);
-            bb = join(bb, item_box(it));
-            cb = join(cb, centroid_box(it));
-        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (j < n) {
+                its[j] = load_item(in + b + j);
+                bb = join(bb, item_box(its[j]));
+                cb = join(cb, centroid_box(its[j]));
+            }
         bb = wave_union(bb);
         if (cnt <= kLeaf || f.depth >= kMaxDepth) {
             if (cnt > 127) {
@@ -471,7 +477,9 @@ __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
             } else if (lane == 0) {
                 region[me] = make_node(bb, f.first - off, true, 0, cnt);
             }
-            for (int i = f.first + lane; i < f.last; i += 64) B.perm[i] = load_item(in + i).id;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j)
+                if (j < n) B.perm[b + j] = its[j].id;
             continue;
         }
         cb = wave_union(cb);
@@ -481,38 +489,62 @@ __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
         int mid;
         int child_buf = f.buf;
         if (extent == 0.0f) {
-            mid = f.first + cnt / 2;
+            mid = f.first + cnt / 2;                       // items stay where they are
         } else {
             const float lo = cb.l[axis];
+            int bin[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) bin[j] = j < n ? bin_of(its[j], axis, lo, extent) : -1;
             DBox bbin[kBins];
             int bn[kBins];
 #pragma unroll
             for (int k = 0; k < kBins; ++k) {
-                DBox acc = empty_box();
-                int n = 0;
-                for (int i = b; i < en; ++i) {
-                    const Item it = load_item(in + i);
-                    if (bin_of(it, axis, lo, extent) == k) { acc = join(acc, item_box(it)); ++n; }
+                bbin[k] = empty_box();
+                bn[k] = 0;
+#pragma unroll
+                for (int j = 0; j < kPer; ++j)
+                    if (bin[j] == k) { bbin[k] = join(bbin[k], item_box(its[j])); ++bn[k]; }
+            }
+            // the twelve ordered lane reductions, interleaved step by step
+#pragma unroll
+            for (int s = 1; s < 64; s <<= 1) {
+#pragma unroll
+                for (int k = 0; k < kBins; ++k) {
+                    DBox o;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        o.l[q] = __shfl_down(bbin[k].l[q], s);
+                        o.h[q] = __shfl_down(bbin[k].h[q], s);
+                    }
+                    bbin[k] = join(bbin[k], o);
+                    bn[k] += __shfl_down(bn[k], s);
                 }
-                bbin[k] = wave_union(acc);
-                bn[k] = wave_sum(n);
+            }
+#pragma unroll
+            for (int k = 0; k < kBins; ++k) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    bbin[k].l[q] = __shfl(bbin[k].l[q], 0);
+                    bbin[k].h[q] = __shfl(bbin[k].h[q], 0);
+                }
+                bn[k] = __shfl(bn[k], 0);
             }
             float lcost[kBins - 1], rcost[kBins - 1];
             DBox acc = empty_box();
-            int n = 0;
+            int c = 0;
 #pragma unroll
             for (int s = 0; s < kBins - 1; ++s) {
                 acc = join(acc, bbin[s]);
-                n += bn[s];
-                lcost[s] = (float)n * area(acc);
+                c += bn[s];
+                lcost[s] = (float)c * area(acc);
             }
             acc = empty_box();
-            n = 0;
+            c = 0;
 #pragma unroll
             for (int s = kBins - 2; s >= 0; --s) {
                 acc = join(acc, bbin[s + 1]);
-                n += bn[s + 1];
-                rcost[s] = (float)n * area(acc);
+                c += bn[s + 1];
+                rcost[s] = (float)c * area(acc);
             }
             const float base = (float)cnt * area(cb);
             float best_cost = kInf;
@@ -524,16 +556,18 @@ __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
             }
             // partition into the other buffer: lefts from first, rights from last - 1 down
             int nl = 0;
-            for (int i = b; i < en; ++i) nl += bin_of(load_item(in + i), axis, lo, extent) <= best;
-            const int nr = (en - b) - nl;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) nl += (j < n && bin[j] <= best) ? 1 : 0;
+            const int nr = n - nl;
             const int tl = wave_sum(nl);
             int l = f.first + wave_excl_scan(nl), r = f.last - 1 - wave_excl_scan(nr);
             Item* out = B.items[f.buf ^ 1];
-            for (int i = b; i < en; ++i) {
-                const Item it = load_item(in + i);
-                if (bin_of(it, axis, lo, extent) <= best) store_item(out + l++, it);
-                else store_item(out + r--, it);
-            }
+#pragma unroll
+            for (int j = 0; j < kPer; ++j)
+                if (j < n) {
+                    if (bin[j] <= best) store_item(out + l++, its[j]);
+                    else store_item(out + r--, its[j]);
+                }
             __threadfence_block();                 // the children read what other lanes wrote
             mid = f.first + tl;
             child_buf = f.buf ^ 1;
@@ -541,43 +575,7 @@ __global__ void __launch_bounds__(kThreads) k_small(Bufs B, int n_small) {
         st[sp++] = {mid, f.last, f.depth + 1, me, child_buf};
         st[sp++] = {f.first, mid, f.depth + 1, -1, child_buf};
     }
-    if (lane == 0) B.region_size[task.first] = nodes;
-}
-
-// Pre-order numbering of each set's upper tree: one thread per set.
-__global__ void k_number(Bufs B, const int* root, int n_sets, int* udfs, int* uright, int* usize, int* rbase,
-                         int* n_nodes) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_sets) return;
-    auto size_of = [&](int code) { return code < 0 ? B.region_size[~code] : usize[code]; };
-    int stk[2 * kMaxDepth + 8], vis[2 * kMaxDepth + 8];
-    int sp = 0;
-    stk[sp] = root[s]; vis[sp++] = 0;
-    while (sp > 0) {                                     // subtree sizes, post-order
-        const int c = stk[sp - 1];
-        if (c < 0) { --sp; continue; }
-        if (!vis[sp - 1]) {
-            vis[sp - 1] = 1;
-            stk[sp] = B.unodes[c].right; vis[sp++] = 0;
-            stk[sp] = B.unodes[c].left; vis[sp++] = 0;
-        } else {
-            --sp;
-            usize[c] = 1 + size_of(B.unodes[c].left) + size_of(B.unodes[c].right);
-        }
-    }
-    n_nodes[s] = size_of(root[s]);
-    sp = 0;
-    stk[sp] = root[s]; vis[sp++] = 0;                    // vis: the pre-order index
-    while (sp > 0) {
-        --sp;
-        const int c = stk[sp], idx = vis[sp];
-        if (c < 0) { rbase[~c] = idx; continue; }
-        const int sl = size_of(B.unodes[c].left);
-        udfs[c] = idx;
-        uright[c] = idx + 1 + sl;
-        stk[sp] = B.unodes[c].right; vis[sp++] = idx + 1 + sl;
-        stk[sp] = B.unodes[c].left; vis[sp++] = idx + 1;
-    }
+    if (lane == 0) B.small_size[ti] = nodes;
 }
 
 __global__ void k_write_upper(Bufs B, int n_unodes, const int* udfs, const int* uright, bn_bvh_node* out) {
@@ -594,7 +592,7 @@ __global__ void __launch_bounds__(kThreads) k_write_regions(Bufs B, int n_small,
     const int ti = blockIdx.x * kWavesPerBlock + w;
     if (ti >= n_small) return;
     const SmallTask t = B.small[ti];
-    const int base = rbase[t.first], n = B.region_size[t.first];
+    const int base = rbase[ti], n = B.small_size[ti];
     const bn_bvh_node* src = B.region + 2 * (size_t)t.first;
     bn_bvh_node* dst = out + 2 * (size_t)B.set_off[t.set] + base;
     for (int k = lane; k < n; k += 64) {
@@ -611,6 +609,20 @@ namespace bnx {
 using namespace bvhg;
 using bnh::Error;
 
+namespace {
+// Page-locked staging memory of the context, grown on demand.
+void* pinned(Ctx* c, size_t bytes) {
+    if (c->g_pinned_bytes < bytes) {
+        if (c->g_pinned) (void)hipHostFree(c->g_pinned);
+        c->g_pinned = nullptr;
+        c->g_pinned_bytes = 0;
+        hip_check(hipHostMalloc(&c->g_pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
+        c->g_pinned_bytes = bytes;
+    }
+    return c->g_pinned;
+}
+}  // namespace
+
 void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vector<int>& counts,
                    std::vector<std::vector<bn_bvh_node>>& nodes_out, std::vector<std::vector<int32_t>>& perm_out) {
     const int S = (int)sets.size();
@@ -624,48 +636,53 @@ void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vec
         off[s + 1] = off[s] + counts[s];
     }
     const int N = off[S];
-    // host: items, finiteness, the level-0 tasks
-    std::vector<Item> items(N);
-    for (int s = 0; s < S; ++s)
+    const int small = std::min(c->bvh_small, kSmallMax);
+    const size_t item_bytes = (size_t)N * sizeof(Item);
+    // host: items into page-locked staging, finiteness, the level-0 tasks
+    Item* items = (Item*)pinned(c, std::max(item_bytes, (size_t)N * (sizeof(bn_bvh_node) * 2 + 4)));
+    for (int s = 0; s < S; ++s) {
+        const float* src = sets[s];
+        Item* dst = items + off[s];
+        bool finite = true;
         for (int i = 0; i < counts[s]; ++i) {
-            const float* b = sets[s] + 6 * (size_t)i;
-            for (int k = 0; k < 6; ++k)
-                if (!std::isfinite(b[k]))
-                    throw Error(BN_E_INVALID, "GPU BVH build needs finite bounds (bn_bvh_build takes any)");
-            items[off[s] + i] = {b[0], b[1], b[2], i, b[3], b[4], b[5], 0};
+            const float* b = src + 6 * (size_t)i;
+            finite = finite && std::isfinite(b[0]) && std::isfinite(b[1]) && std::isfinite(b[2]) &&
+                     std::isfinite(b[3]) && std::isfinite(b[4]) && std::isfinite(b[5]);
+            dst[i] = {b[0], b[1], b[2], i, b[3], b[4], b[5], 0};
         }
-    const int big_cap = N / kSmall + S + 1;
+        if (!finite) throw Error(BN_E_INVALID, "GPU BVH build needs finite bounds (bn_bvh_build takes any)");
+    }
+    const int big_cap = N / small + S + 1;
     const int unode_cap = kMaxDepth * big_cap;
-    const int small_cap = std::min((int64_t)N + S, 2 * (int64_t)unode_cap + S) + 1;
+    const int small_cap = (int)std::min((int64_t)N + S, 2 * (int64_t)unode_cap + S) + 1;
     const int chunk_cap = 2 * (N / kChunk + big_cap) + 1;
     std::vector<BigTask> big0;
     std::vector<SmallTask> small0;
     std::vector<int> root(S), ctask, cstart;
     for (int s = 0; s < S; ++s) {
-        if (counts[s] > kSmall) {
+        if (counts[s] > small) {
             const int u = (int)big0.size();
             const int nc = (counts[s] + kChunk - 1) / kChunk;
             big0.push_back({off[s], off[s + 1], 0, u, s, (int)ctask.size(), nc, 0});
             for (int k = 0; k < nc; ++k) { ctask.push_back(u); cstart.push_back(off[s] + k * kChunk); }
             root[s] = u;
         } else {
+            root[s] = ~(int)small0.size();
             small0.push_back({off[s], off[s + 1], 0, 0, s, 0, 0, 0});
-            root[s] = ~off[s];
         }
     }
     // device memory (grow-only buffers of the context)
-    const size_t item_bytes = (size_t)N * sizeof(Item);
     c->g_items.ensure(2 * std::max(item_bytes, (size_t)N * sizeof(bn_bvh_node)));   // A | B, then the nodes
     c->g_region.ensure(2 * (size_t)N * sizeof(bn_bvh_node));
     c->g_perm.ensure((size_t)N * 4);
-    c->g_rsize.ensure((size_t)N * 4 * 2);                // region sizes, region bases
+    c->g_rsize.ensure((size_t)small_cap * 4 * 2);         // region sizes, region bases (by small task)
     c->g_big.ensure(2 * (size_t)big_cap * sizeof(BigTask));
     c->g_info.ensure((size_t)big_cap * sizeof(BigInfo));
     c->g_small.ensure((size_t)small_cap * sizeof(SmallTask));
     c->g_chunks.ensure(4 * (size_t)chunk_cap * 4);
     c->g_part.ensure((size_t)chunk_cap * (2 + kBins) * sizeof(DBox) + (size_t)chunk_cap * (kBins + 2) * 4);
-    c->g_unodes.ensure((size_t)unode_cap * (sizeof(UNode) + 3 * 4));
-    c->g_misc.ensure(sizeof(Ctl) + (size_t)S * 4 * 3);
+    c->g_unodes.ensure((size_t)unode_cap * (sizeof(UNode) + 2 * 4));
+    c->g_misc.ensure(sizeof(Ctl) + (size_t)S * 4);
     hipStream_t st = c->stream;
     char* items_base = (char*)c->g_items.p;
     Bufs B{};
@@ -685,19 +702,16 @@ void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vec
     B.unodes = (UNode*)c->g_unodes.p;
     int* udfs = (int*)(B.unodes + unode_cap);
     int* uright = udfs + unode_cap;
-    int* usize = uright + unode_cap;
     B.region = (bn_bvh_node*)c->g_region.p;
-    B.region_size = (int*)c->g_rsize.p;
-    int* rbase = B.region_size + N;
+    B.small_size = (int*)c->g_rsize.p;
+    int* rbase = B.small_size + small_cap;
     B.perm = (int32_t*)c->g_perm.p;
     B.ctl = (Ctl*)c->g_misc.p;
     int* d_off = (int*)((char*)c->g_misc.p + sizeof(Ctl));
-    int* d_root = d_off + S;
-    int* d_nn = d_root + S;
     B.set_off = d_off;
     B.big_cap = big_cap; B.small_cap = small_cap; B.unode_cap = unode_cap; B.chunk_cap = chunk_cap;
 
-    hip_check(hipMemcpyAsync(B.items[0], items.data(), item_bytes, hipMemcpyHostToDevice, st), "bvh items");
+    hip_check(hipMemcpyAsync(B.items[0], items, item_bytes, hipMemcpyHostToDevice, st), "bvh items");
     if (!big0.empty()) {
         hip_check(hipMemcpyAsync(bigs[0], big0.data(), big0.size() * sizeof(BigTask), hipMemcpyHostToDevice, st), "bvh tasks");
         hip_check(hipMemcpyAsync(chunks[0], ctask.data(), ctask.size() * 4, hipMemcpyHostToDevice, st), "bvh chunks");
@@ -707,7 +721,6 @@ void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vec
         hip_check(hipMemcpyAsync(B.small, small0.data(), small0.size() * sizeof(SmallTask), hipMemcpyHostToDevice, st),
                   "bvh small tasks");
     hip_check(hipMemcpyAsync(d_off, off.data(), S * 4, hipMemcpyHostToDevice, st), "bvh sets");
-    hip_check(hipMemcpyAsync(d_root, root.data(), S * 4, hipMemcpyHostToDevice, st), "bvh roots");
     Ctl ctl{};
     ctl.n_small = (int)small0.size();
     ctl.n_unodes = (int)big0.size();
@@ -728,7 +741,7 @@ void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vec
         k_chunk_bounds<<<n_chunks, kThreads, 0, st>>>(B, src);
         k_task_bounds<<<tb, 64, 0, st>>>(B, n_big);
         k_chunk_bins<<<n_chunks, kThreads, 0, st>>>(B, src);
-        k_task_split<<<tb, 64, 0, st>>>(B, n_big, level);
+        k_task_split<<<tb, 64, 0, st>>>(B, n_big, level, small);
         k_chunk_scatter<<<n_chunks, kThreads, 0, st>>>(B, src);
         k_plan<<<1, kThreads, 0, st>>>(B);
         hip_check(hipGetLastError(), "bvh level launch");
@@ -742,28 +755,73 @@ void bvh_build_gpu(Ctx* c, const std::vector<const float*>& sets, const std::vec
     const int n_small = ctl.n_small;
     const int n_unodes = ctl.n_unodes;
     if (n_small > 0) k_small<<<(n_small + kWavesPerBlock - 1) / kWavesPerBlock, kThreads, 0, st>>>(B, n_small);
-    bn_bvh_node* out = (bn_bvh_node*)items_base;                // the item buffers are done with
-    k_number<<<(S + 63) / 64, 64, 0, st>>>(B, d_root, S, udfs, uright, usize, rbase, d_nn);
-    if (n_unodes > 0) k_write_upper<<<(n_unodes + 255) / 256, 256, 0, st>>>(B, n_unodes, udfs, uright, out);
-    if (n_small > 0)
-        k_write_regions<<<(n_small + kWavesPerBlock - 1) / kWavesPerBlock, kThreads, 0, st>>>(B, n_small, rbase, out);
-    hip_check(hipGetLastError(), "bvh flatten launch");
-    std::vector<int> nn(S);
+    hip_check(hipGetLastError(), "bvh small launch");
+    // Pre-order numbering of the upper trees on the host (a few nodes per
+    // 512 items): subtree sizes, then each upper node's index and right
+    // child and each region's base.
+    std::vector<UNode> un(n_unodes);
+    std::vector<SmallTask> sm(n_small);
+    std::vector<int> rsize(n_small);
+    if (n_unodes) hip_check(hipMemcpyAsync(un.data(), B.unodes, n_unodes * sizeof(UNode), hipMemcpyDeviceToHost, st), "bvh upper");
+    hip_check(hipMemcpyAsync(sm.data(), B.small, n_small * sizeof(SmallTask), hipMemcpyDeviceToHost, st), "bvh small");
+    hip_check(hipMemcpyAsync(rsize.data(), B.small_size, n_small * 4, hipMemcpyDeviceToHost, st), "bvh sizes");
     hip_check(hipMemcpyAsync(&ctl, B.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st), "bvh ctl");
-    hip_check(hipMemcpyAsync(nn.data(), d_nn, S * 4, hipMemcpyDeviceToHost, st), "bvh node counts");
-    hip_check(hipStreamSynchronize(st), "bvh build");
+    hip_check(hipStreamSynchronize(st), "bvh subtrees");
     if (ctl.err & 1) throw Error(BN_E_CAPACITY, "BVH leaf larger than the int8 count field");
     if (ctl.err) throw Error(BN_E_CAPACITY, "GPU BVH build: task capacity exceeded");
+    std::vector<int> usize(n_unodes), h_udfs(n_unodes), h_uright(n_unodes), h_rbase(n_small), nn(S);
+    auto size_of = [&](int code) { return code < 0 ? rsize[~code] : usize[code]; };
+    std::vector<std::pair<int, int>> stk;
     for (int s = 0; s < S; ++s) {
+        stk.assign(1, {root[s], 0});
+        while (!stk.empty()) {                               // post-order: sizes
+            auto& [code, vis] = stk.back();
+            if (code < 0) { stk.pop_back(); continue; }
+            const int u = code;
+            if (!vis) {
+                vis = 1;
+                stk.push_back({un[u].right, 0});
+                stk.push_back({un[u].left, 0});
+            } else {
+                stk.pop_back();
+                usize[u] = 1 + size_of(un[u].left) + size_of(un[u].right);
+            }
+        }
+        nn[s] = size_of(root[s]);
         if (nn[s] <= 0 || nn[s] > 2 * counts[s]) throw Error(BN_E_DEVICE, "GPU BVH build: inconsistent node count");
-        nodes_out[s].resize(nn[s]);
-        perm_out[s].resize(counts[s]);
-        hip_check(hipMemcpyAsync(nodes_out[s].data(), out + 2 * (size_t)off[s], nn[s] * sizeof(bn_bvh_node),
-                                 hipMemcpyDeviceToHost, st), "bvh nodes");
-        hip_check(hipMemcpyAsync(perm_out[s].data(), B.perm + off[s], counts[s] * 4, hipMemcpyDeviceToHost, st),
-                  "bvh perm");
+        stk.assign(1, {root[s], 0});
+        while (!stk.empty()) {                               // pre-order: indices
+            const auto [code, idx] = stk.back();
+            stk.pop_back();
+            if (code < 0) { h_rbase[~code] = idx; continue; }
+            const int sl = size_of(un[code].left);
+            h_udfs[code] = idx;
+            h_uright[code] = idx + 1 + sl;
+            stk.push_back({un[code].right, idx + 1 + sl});
+            stk.push_back({un[code].left, idx + 1});
+        }
     }
+    bn_bvh_node* out = (bn_bvh_node*)items_base;                // the item buffers are done with
+    if (n_unodes) {
+        hip_check(hipMemcpyAsync(udfs, h_udfs.data(), n_unodes * 4, hipMemcpyHostToDevice, st), "bvh numbering");
+        hip_check(hipMemcpyAsync(uright, h_uright.data(), n_unodes * 4, hipMemcpyHostToDevice, st), "bvh numbering");
+        k_write_upper<<<(n_unodes + 255) / 256, 256, 0, st>>>(B, n_unodes, udfs, uright, out);
+    }
+    hip_check(hipMemcpyAsync(rbase, h_rbase.data(), n_small * 4, hipMemcpyHostToDevice, st), "bvh numbering");
+    k_write_regions<<<(n_small + kWavesPerBlock - 1) / kWavesPerBlock, kThreads, 0, st>>>(B, n_small, rbase, out);
+    hip_check(hipGetLastError(), "bvh flatten launch");
+    // results through the staging memory: the nodes of every set, then the permutation
+    bn_bvh_node* h_nodes = (bn_bvh_node*)items;
+    int32_t* h_perm = (int32_t*)(h_nodes + 2 * (size_t)N);
+    for (int s = 0; s < S; ++s)
+        hip_check(hipMemcpyAsync(h_nodes + 2 * (size_t)off[s], out + 2 * (size_t)off[s], nn[s] * sizeof(bn_bvh_node),
+                                 hipMemcpyDeviceToHost, st), "bvh nodes");
+    hip_check(hipMemcpyAsync(h_perm, B.perm, (size_t)N * 4, hipMemcpyDeviceToHost, st), "bvh perm");
     hip_check(hipStreamSynchronize(st), "bvh copy");
+    for (int s = 0; s < S; ++s) {
+        nodes_out[s].assign(h_nodes + 2 * (size_t)off[s], h_nodes + 2 * (size_t)off[s] + nn[s]);
+        perm_out[s].assign(h_perm + off[s], h_perm + off[s + 1]);
+    }
 }
 
 }  // namespace bnx

@@ -132,9 +132,18 @@ def batched(ctx, n=32, pause_s=0.0, small=True):
     so that a kernel trace of the run splits into them at its longest gaps."""
     t0 = time.perf_counter()
     scs = [scene.load_scene(j) for j in configs.config2(n)]
-    ds = [ctx.scene(s) for s in scs]
+    load_s = time.perf_counter() - t0
+    ctx.set_option("bvh_gpu", 0)                          # the same scene builds with every BVH on the host
+    t0 = time.perf_counter()
+    for s in scs:
+        ctx.scene(s).release()
+    host_build_s = time.perf_counter() - t0
+    ctx.set_option("bvh_gpu", 1)
+    t0 = time.perf_counter()
+    ds = [ctx.scene(s) for s in scs]                      # BLAS built on the GPU
+    build_s = time.perf_counter() - t0
+    setup = load_s + build_s
     jobs = [(d, s.pt_params()) for d, s in zip(ds, scs)]
-    setup = time.perf_counter() - t0
     for d, p in jobs:
         d.render_pt(p)
     time.sleep(pause_s)
@@ -154,7 +163,42 @@ def batched(ctx, n=32, pause_s=0.0, small=True):
                       "256^2-1024^2, 4-64 spp, depth 4-16",
             "paths": paths, "plain_s": plain_s, "batched_s": batch_s, "speedup": plain_s / batch_s,
             "batched_mpaths_per_s": paths / batch_s / 1e6, "rounds": st["rounds"], "graph": st["graph"],
-            "films_bit_equal": equal, "scene_setup_s": setup, "small_items": small_items(ctx) if small else None}
+            "films_bit_equal": equal, "scene_setup_s": setup, "scene_load_s": load_s, "scene_build_s": build_s,
+            "scene_build_host_bvh_s": host_build_s,
+            "small_items": small_items(ctx) if small else None, "_scenes": scs}
+
+
+def bvh_builds(ctx, scs):
+    """BLAS construction (BVHNode.Build over triangle bounds, Mesh.fs:159-169)
+    of every mesh of the given scenes: the host build and the GPU build, one
+    call per mesh each (the GPU time includes copying the bounds in and the
+    nodes and permutation out); nodes and permutations must be equal."""
+    sets = []
+    for sc in scs:
+        v = sc.vertices.reshape(-1, 3)
+        for pr in sc.primitives:
+            if not pr.get("n_indices"):
+                continue
+            ix = sc.indices[pr["index_offset"]:pr["index_offset"] + pr["n_indices"]].reshape(-1, 3)
+            tri = v[pr["vert_offset"]:pr["vert_offset"] + pr["n_verts"]][ix]
+            sets.append(np.ascontiguousarray(np.concatenate([tri.min(1), tri.max(1)], 1), np.float32))
+    ctx.bvh_build(sets[0])                                # allocation outside the timing
+    host_t, gpu_t, equal = [], [], True
+    for b in sets:
+        t0 = time.perf_counter()
+        hn, hp = native.bvh_build(b)
+        host_t.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        gn, gp = ctx.bvh_build(b)
+        gpu_t.append(time.perf_counter() - t0)
+        equal = equal and gn.tobytes() == hn.tobytes() and np.array_equal(gp, hp)
+    tris = [len(b) for b in sets]
+    k = int(np.argmax(tris))
+    return {"meshes": len(sets), "triangles": int(sum(tris)), "host_s": float(sum(host_t)),
+            "gpu_s": float(sum(gpu_t)), "host_mtri_per_s": sum(tris) / sum(host_t) / 1e6,
+            "gpu_mtri_per_s": sum(tris) / sum(gpu_t) / 1e6, "nodes_and_perm_equal": bool(equal),
+            "largest": {"triangles": tris[k], "host_ms": host_t[k] * 1e3, "gpu_ms": gpu_t[k] * 1e3},
+            "host": "one thread (the reference's build is sequential too)"}
 
 
 def timeline_summary(csv_path, min_phase_gap_s=0.2):
@@ -418,7 +462,7 @@ def main():
     if args.batch_timeline:
         ctx = native.Context(0)
         r = batched(ctx, pause_s=0.5, small=False)
-        print(json.dumps({k: v for k, v in r.items() if k != "small_items"}), flush=True)
+        print(json.dumps({k: v for k, v in r.items() if k not in ("small_items", "_scenes")}), flush=True)
         return
 
     sc = scene.load_scene(configs.config1(spp=args.spp))
@@ -510,6 +554,7 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched(ctx)
+        out["bvh_build"] = bvh_builds(ctx, out["batched"].pop("_scenes"))
     if not args.no_adjoint:
         out["adjoint"] = adjoint(ctx)
     if not args.no_pssmlt:
