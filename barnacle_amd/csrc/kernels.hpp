@@ -51,11 +51,6 @@ struct PathState {
     uint32_t* pid;
 };
 
-// Target of a compact round's shadow ray (in nee_a.w): the position of its
-// path in the next state copy, or kTargetDone | path id when the path has
-// ended and its radiance already sits in the per-id output.
-constexpr uint32_t kTargetDone = 0x80000000u;
-
 // Device-side counters of the iterations of a round.
 struct Ctl {
     uint32_t q_count[2];      // active queues (this iteration, next iteration)

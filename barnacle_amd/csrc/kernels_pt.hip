@@ -188,10 +188,11 @@ __global__ __launch_bounds__(256) void k_regen(Work<BATCH> W, PathBuf P, const C
 
 // Compact rounds: the new paths take positions after the n_act paths that
 // continue from the previous iteration, up to n_slots, with the seeding and
-// primary rays of k_regen.  Ended paths were retired by the shading kernels.
+// primary rays of k_regen.  A path's radiance is accumulated in L_out[path
+// id] for the whole round, from zero here.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_regen_c(Work<BATCH> W, PathBuf X, const Ctl* ctl, int cur,
-                                                 uint32_t n_slots) {
+                                                 uint32_t n_slots, float4* L_out) {
     const uint32_t n_act = ctl->q_count[cur];
     const uint32_t next_pid = ctl->next_pid;
     const uint32_t n_new = new_paths(n_slots - n_act, next_pid, W.total_paths());
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(256) void k_regen_c(Work<BATCH> W, PathBuf X, const
             X.o[pos] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
             X.d[pos] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         }
-        X.L[pos] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        L_out[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         X.pid[pos] = id;
         if (BATCH) X.item[pos] = k;
     }
@@ -495,15 +496,17 @@ struct Vertex {
     uint32_t rng;
     float4 sh_o, sh_d, nee_a, nee_b;  // shadow ray (sh_o.w t max) and NEE factors, when shadow
 };
+// Lacc: the path's radiance accumulator (compact rounds), read and written
+// only when the vertex emits; null: the radiance is P.L[at], returned in v.L.
 __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem& it, const PathBuf& P, uint32_t at,
-                                             int4 h, Vertex& v) {
+                                             int4 h, Vertex& v, float4* Lacc = nullptr) {
     const float4 o4 = P.o[at], d4 = P.d[at];
     const Ray r = {xyz(o4), xyz(d4)};
     const float prev_pdf = d4.w;
     const int depth = __float_as_int(o4.w);
     const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
     const int4 info = S.insts[h.x].info;
-    V3 L = xyz(P.L[at]);
+    V3 L = Lacc ? V3{0.0f, 0.0f, 0.0f} : xyz(P.L[at]);
     V3 beta = xyz(P.beta[at]);
     v.cont = false;
     v.shadow = false;
@@ -512,7 +515,8 @@ __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem&
         float lpdf;
         light_eval(S, h.x, r.o, g, Le, lpdf);
         const float w = depth == 0 ? 1.0f : prev_pdf * rcp_e(lpdf + prev_pdf);
-        L = fma3(beta, Le * w, L);
+        if (Lacc) *Lacc = f4(fma3(beta, Le * w, xyz(*Lacc)), 0.0f);
+        else L = fma3(beta, Le * w, L);
     }
     if (info.z >= 0) {
         uint32_t rng = P.rng[at];
@@ -600,16 +604,17 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 
 // Compact rounds: position i of the current copy X for every active path
 // (misses included: they end here).  A path that continues moves to its
-// compacted position in the other copy Y; one that ends leaves its radiance
-// in L_out[path id].  Shadow rays are stored by their own position, with the
-// item (batch) in sh_d.w and the target of the NEE term in nee_a.w.  The
-// vertices shaded are counted in bin_count[0].
+// compacted position in the other copy Y.  Radiance accumulates in L_out[path
+// id] (emission here, NEE terms in k_trace_shadow).  Shadow rays are stored
+// by their own position, with the item (batch) in sh_d.w and the path id in
+// nee_a.w.  The vertices shaded are counted in bin_count[0].
 // The vertex results wait in LDS while the block computes the output
 // positions, so they do not occupy registers across that step (the kernel
 // would otherwise need about 160 VGPRs, three waves per SIMD).
 struct ShadeStage {
-    float4 o[256], d[256], beta[256], L[256];     // L.w: rng bits
+    float4 o[256], d[256], beta[256];
     float4 sh_o[256], sh_d[256], nee_a[256], nee_b[256];
+    uint32_t rng[256];
 };
 template <bool BATCH>
 __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
@@ -632,13 +637,11 @@ __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, Pa
             v.shadow = false;
             if (h.x >= 0) {
                 ++shaded;
-                shade_vertex(W.scene(k), W.item(k), X, i, h, v);
-            } else {
-                v.L = xyz(X.L[i]);
+                shade_vertex(W.scene(k), W.item(k), X, i, h, v, &L_out[pid]);
             }
             cont = v.cont;
             shadow = v.shadow;
-            sg.L[t] = f4(v.L, __uint_as_float(v.rng));
+            sg.rng[t] = v.rng;
             if (cont) {
                 sg.o[t] = v.o;
                 sg.d[t] = v.d;
@@ -654,25 +657,21 @@ __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, Pa
         uint32_t pos[2];
         block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->sh_count}, pos);
         if (i < total) {
-            const float4 l = sg.L[t];
             if (cont) {
                 const uint32_t j = pos[0];
                 Y.o[j] = sg.o[t];
                 Y.d[j] = sg.d[t];
                 Y.beta[j] = sg.beta[t];
-                Y.L[j] = make_float4(l.x, l.y, l.z, 0.0f);
-                Y.rng[j] = __float_as_uint(l.w);
+                Y.rng[j] = sg.rng[t];
                 Y.pid[j] = pid;
                 if (BATCH) Y.item[j] = k;
-            } else {
-                L_out[pid] = make_float4(l.x, l.y, l.z, 0.0f);
             }
             if (shadow) {
                 const uint32_t q = pos[1];
                 const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
                 X.sh_o[q] = sg.sh_o[t];
                 X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
-                X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(cont ? pos[0] : (kTargetDone | pid)));
+                X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(pid));
                 X.nee_b[q] = sg.nee_b[t];
             }
         }
@@ -685,10 +684,10 @@ __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, Pa
 // aggregate.Intersect(shadowRay, lightDistance - 1e-3) (PathTracing.fs:47-59);
 // an unoccluded ray adds beta*f * Le/(pdf_b + pdf_l) to L.  Slot rounds: the
 // rays of the slots in shq, added to the slot's L.  Compact rounds (shq
-// null): ray i, added to L_next[target] or to L_out[path id] (nee_a.w).
+// null): ray i, added to L_out[path id] (nee_a.w).
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWaves) : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
-                                                      float4* L_next, float4* L_out,
+                                                      float4* L_out,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
@@ -712,11 +711,7 @@ __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWa
                 const uint32_t slot = shq ? shq[i] : i;
                 const float4 a = P.nee_a[slot];
                 const V3 b = xyz(P.nee_b[slot]);
-                float4* dst = &P.L[slot];
-                if (!shq) {
-                    const uint32_t t = __float_as_uint(a.w);
-                    dst = (t & kTargetDone) ? &L_out[t & ~kTargetDone] : &L_next[t];
-                }
+                float4* dst = shq ? &P.L[slot] : &L_out[__float_as_uint(a.w)];
                 *dst = f4(fma3(xyz(a), b, xyz(*dst)), 0.0f);
             }
         });
@@ -1222,11 +1217,10 @@ static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const Path
 }
 template <bool BATCH>
 static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                            float4* L_next, float4* L_out, const Buffers& B, uint32_t refill, int grid,
-                            hipStream_t st) {
+                            float4* L_out, const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
 #define BN_TS(C, KK)                                                                                            \
-    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_next,  \
-                       L_out, B.spill, B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
+    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_out,   \
+                       B.spill, B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
     if (k == 16) { if (count) BN_TS(true, 16); else BN_TS(false, 16); }
     else { if (count) BN_TS(true, 32); else BN_TS(false, 32); }
 #undef BN_TS
@@ -1277,7 +1271,8 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
         const PathBuf& X = copies[cur];
         const PathBuf& Y = copies[nxt];
         { Bracket b(hook, "regen", st);
-          hipLaunchKernelGGL(k_regen_c<BATCH>, dim3(grid_c), dim3(256), 0, st, W, X, ctl, cur, L.n_slots); }
+          hipLaunchKernelGGL(k_regen_c<BATCH>, dim3(grid_c), dim3(256), 0, st, W, X, ctl, cur, L.n_slots,
+                             B.L_out); }
         { Bracket b(hook, "prep", st);
           hipLaunchKernelGGL(k_prep<BATCH>, dim3(1), dim3(2 * kFetchSegs), 0, st, W, ctl, cur, B.fetch_seg,
                              L.n_slots); }
@@ -1292,10 +1287,10 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
                              nxt, B.L_out); }
         { Bracket b(hook, "trace_shadow", st);
           if (one_scene)
-              launch_shadow_t<false>(count, B.stack_k, plain_work(L), X, ctl, nullptr, Y.L, B.L_out, B, L.refill,
+              launch_shadow_t<false>(count, B.stack_k, plain_work(L), X, ctl, nullptr, B.L_out, B, L.refill,
                                      L.grid_uniform, st);
           else
-              launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, Y.L, B.L_out, B, L.refill, L.grid_trace,
+              launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, B.L_out, B, L.refill, L.grid_trace,
                                      st); }
         if (hook) hook->after_bounce(cur);
     }
@@ -1337,7 +1332,7 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
                   hipLaunchKernelGGL(k_shade_direct, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins,
                                      B.bin_cap, B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
                 { Bracket b(hook, "trace_shadow", st);
-                  launch_shadow_t<false>(count, B.stack_k, W, P, ctl, B.shq, nullptr, nullptr, B, L.refill,
+                  launch_shadow_t<false>(count, B.stack_k, W, P, ctl, B.shq, nullptr, B, L.refill,
                                          L.grid_trace, st); }
             }
         } else if (kind == Integrator::Adjoint) {
@@ -1376,7 +1371,7 @@ static void iterations_t(const Work<BATCH>& W, Integrator kind, const Launch& L,
               hipLaunchKernelGGL(k_shade<BATCH>, dim3(L.grid_shade), dim3(256), 0, st, W, P, ctl, B.bins, B.bin_cap,
                                  B.queue[nxt], nxt, B.shq, B.freel[nxt]); }
             { Bracket b(hook, "trace_shadow", st);
-              launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, nullptr, nullptr, B, L.refill, L.grid_trace,
+              launch_shadow_t<BATCH>(count, B.stack_k, W, P, ctl, B.shq, nullptr, B, L.refill, L.grid_trace,
                                      st); }
         }
         if (hook) hook->after_bounce(cur);
@@ -1439,7 +1434,7 @@ void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* c
     Work<false> W;
     W.S = S;
     W.it = RoundItem{};
-    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, nullptr, nullptr, B, refill, grid, st);
+    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, nullptr, B, refill, grid, st);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {
