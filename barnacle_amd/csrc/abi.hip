@@ -354,9 +354,10 @@ bnd::PathBuf path_buf(Ctx* c) {
     return P;
 }
 
+// Two spill areas: a shadow traversal may run beside trace_closest.
 void ensure_spill(Ctx* c) {
     const size_t lanes = (size_t)grid_trace(c) * bnd::kBlock;
-    const size_t need = lanes * (bnd::kStackMax - bnd::kStackKMin) * 4;
+    const size_t need = 2 * lanes * (bnd::kStackMax - bnd::kStackKMin) * 4;
     if (need > c->spill.bytes) c->graphs.clear();
     c->spill.ensure(need);
     c->spill_stride = lanes;
@@ -372,6 +373,7 @@ bnd::Buffers buffers(Ctx* c) {
     B.bins = (uint32_t*)c->bins.p;
     B.bin_cap = (uint32_t)c->path_cap;
     B.spill = (uint32_t*)c->spill.p;
+    B.spill2 = B.spill + c->spill.bytes / 8;              // the second half
     B.spill_stride = c->spill_stride;
     B.stack_k = c->stack_k;
     B.stats = (unsigned long long*)c->stats.p;
@@ -399,7 +401,7 @@ struct Hook : bnd::LaunchHook {
         hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream), "ctl readback");
         hip_check(hipStreamSynchronize(c->stream), "sync");
         acc->rays_closest += h.q_count[cur];
-        acc->rays_shadow += h.sh_count;
+        acc->rays_shadow += h.sh_count + h.shc[cur];     // slot rounds, compact rounds
         for (int b = 0; b < bnd::kBins; ++b) acc->vertices += h.bin_count[b];
     }
 };
@@ -497,6 +499,11 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bnd::Integrator 
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.stream = s;
+    if (c->overlap_shadow) {
+        L.stream2 = c->stream2;
+        L.ev_shade = c->ev_shade;
+        L.ev_shadow = c->ev_shadow;
+    }
     L.n_pix = (uint32_t)(p.width * p.height);
     L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
@@ -1084,6 +1091,9 @@ int bn_create(int device, bn_ctx** out) {
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->n_cu = prop.multiProcessorCount;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_shade, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_shadow, hipEventDisableTiming), "hipEventCreate");
         noalloc_init(device);
         *out = reinterpret_cast<bn_ctx*>(c.release());
     });
@@ -1456,6 +1466,9 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "bvh_gpu") {
             if (value < 0 || value > 1) throw Error(BN_E_INVALID, "bvh_gpu must be 0 or 1");
             c->bvh_gpu = value != 0;
+        } else if (k == "overlap_shadow") {
+            if (value < 0 || value > 1) throw Error(BN_E_INVALID, "overlap_shadow must be 0 or 1");
+            c->overlap_shadow = value != 0;
         } else if (k == "bvh_small") {
             if (value < 128 || value > 512) throw Error(BN_E_INVALID, "bvh_small must be in [128, 512]");
             c->bvh_small = (int)value;

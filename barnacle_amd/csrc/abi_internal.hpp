@@ -80,6 +80,9 @@ struct Ctx {
     int device = 0;
     int n_cu = 1;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;        // shadow rays of compact rounds, beside `stream`
+    hipEvent_t ev_shade = nullptr, ev_shadow = nullptr;
+    bool overlap_shadow = true;           // trace shadow rays on stream2 in plain path-tracing frames
     int trace_blocks_per_cu = 7;
     int refill = 32;                      // idle lanes that trigger a refill
     size_t chunk_slots = (size_t)64 << 20;       // path slots in flight
@@ -124,6 +127,9 @@ struct Ctx {
     std::vector<int32_t> last_chain_accepts;
     ~Ctx() {
         if (g_pinned) (void)hipHostFree(g_pinned);
+        if (ev_shade) (void)hipEventDestroy(ev_shade);
+        if (ev_shadow) (void)hipEventDestroy(ev_shadow);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

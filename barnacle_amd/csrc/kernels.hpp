@@ -60,6 +60,7 @@ struct Ctl {
     uint32_t free_count[2];   // free-slot lists (read by this iteration's regen, filled for the next)
     uint32_t next_pid;        // next path id of the round to issue (may run past the end)
     uint32_t pad;
+    uint32_t shc[2];          // compact rounds: shadow rays of the iterations of each parity
 };
 
 // One render item of a round: samples [s0, s0 + ns) of every pixel of one
@@ -101,6 +102,12 @@ struct Launch {
     int grid_uniform;
     uint32_t refill;
     hipStream_t stream;
+    // Compact path-tracing rounds with stream2 set trace the shadow rays of
+    // iteration i on stream2 while stream runs regen, prep and trace_closest
+    // of iteration i + 1; ev_shade / ev_shadow order the two streams.  Not
+    // for rounds captured into a graph.
+    hipStream_t stream2;
+    hipEvent_t ev_shade, ev_shadow;
 };
 
 // Path-replay adjoint state (section "adjoint" of kernels_pt.hip).
@@ -125,12 +132,14 @@ struct Buffers {
     uint32_t* bins;
     uint32_t bin_cap;
     uint32_t* spill;
+    uint32_t* spill2;         // a second spill area, for a shadow traversal running beside trace_closest
     size_t spill_stride;
     int stack_k;              // LDS stack window: 16 or 32 entries
     unsigned long long* stats;
     float* film;
     float4* L_out;            // radiance per path id of the round
-    uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters (closest, shadow)
+    uint32_t* fetch_seg;      // 2 x kFetchSegs fetch counters: closest, then the shadow
+                              // traversal (compact rounds: two sets, one per parity)
     int sort;                 // bin the shade queue by material kind (1), not (0), adjoint only (-1)
     AdjBuf adj;               // used by Integrator::Adjoint only
     PathState alt;            // second state copy: Integrator::Path rounds run compact when it is set

@@ -227,11 +227,19 @@ __global__ __launch_bounds__(256) void k_regen_c(Work<BATCH> W, PathBuf X, const
 // counters of the other parity (filled during this iteration) start at zero.
 // compact_slots: 0 for slot rounds (new paths fill the free list), else the
 // capacity of a compact round.
+// Compact rounds clear only this parity's shadow counters: the shadow rays
+// of the previous iteration may still be in flight with the other set.
 template <bool BATCH>
 __global__ void k_prep(Work<BATCH> W, Ctl* ctl, int cur, uint32_t* fetch_seg, uint32_t compact_slots) {
     const int i = threadIdx.x;
     const int nxt = 1 - cur;
-    if (i < 2 * kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
+    if (compact_slots) {
+        if (i < kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
+        else if (i < kFetchSegs + kShadowSegs)
+            fetch_seg[(size_t)(i + cur * kShadowSegs) * kSegStride] = 0;
+    } else if (i < 2 * kFetchSegs) {
+        fetch_seg[(size_t)i * kSegStride] = 0;
+    }
     if (i < kBins) ctl->bin_count[i] = 0;
     if (i == 0) {
         const uint32_t n_free = compact_slots ? compact_slots - ctl->q_count[cur] : ctl->free_count[cur];
@@ -239,6 +247,7 @@ __global__ void k_prep(Work<BATCH> W, Ctl* ctl, int cur, uint32_t* fetch_seg, ui
         ctl->q_count[cur] += n_new;
         ctl->next_pid += n_new;
         ctl->sh_count = 0;
+        ctl->shc[cur] = 0;
         ctl->q_count[nxt] = 0;
         ctl->free_count[nxt] = 0;
     }
@@ -655,7 +664,7 @@ __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, Pa
             }
         }
         uint32_t pos[2];
-        block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->sh_count}, pos);
+        block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->shc[cur]}, pos);
         if (i < total) {
             if (cont) {
                 const uint32_t j = pos[0];
@@ -687,11 +696,11 @@ __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, Pa
 // null): ray i, added to L_out[path id] (nee_a.w).
 template <bool COUNT, bool BATCH, int K>
 __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWaves) : 1) void k_trace_shadow(Work<BATCH> W, PathBuf P, Ctl* ctl, const uint32_t* shq,
-                                                      float4* L_out,
+                                                      float4* L_out, int par,
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
-    const uint32_t count = ctl->sh_count;
+    const uint32_t count = shq ? ctl->sh_count : ctl->shc[par];
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT, K>(
         SegFetch{seg, kShadowSegs}, count, stk, spill, spill_stride, refill, tc,
@@ -1215,12 +1224,17 @@ static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const Path
     else { if (count) BN_TC(true, 32); else BN_TC(false, 32); }
 #undef BN_TC
 }
+// par: the parity of a compact round's iteration (its shadow count and fetch
+// counters); spill: the stack spill area to use.
 template <bool BATCH>
 static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                            float4* L_out, const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
+                            float4* L_out, const Buffers& B, uint32_t refill, int grid, hipStream_t st, int par = 0,
+                            uint32_t* spill = nullptr) {
+    uint32_t* seg = B.fetch_seg + (size_t)(kFetchSegs + par * kShadowSegs) * kSegStride;
+    if (!spill) spill = B.spill;
 #define BN_TS(C, KK)                                                                                            \
-    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_out,   \
-                       B.spill, B.spill_stride, refill, B.stats, B.fetch_seg + kFetchSegs * kSegStride)
+    hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_out, par, \
+                       spill, B.spill_stride, refill, B.stats, seg)
     if (k == 16) { if (count) BN_TS(true, 16); else BN_TS(false, 16); }
     else { if (count) BN_TS(true, 32); else BN_TS(false, 32); }
 #undef BN_TS
@@ -1266,6 +1280,13 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
     // bounce, which reads the positions in the order the blocks appended
     // them, was measured 1.6% (config 1) to 3.6% (32-scene batch) slower.
     const int grid_c = L.grid_shade / 2;
+    // With a second stream, the shadow rays of iteration i are traced while
+    // regen, prep and trace_closest of iteration i + 1 run: shade_c(i + 1)
+    // waits for trace_shadow(i) (it reuses the shadow arrays and adds to the
+    // same radiance slots), and the block ends with both streams joined.
+    const bool overlap = L.stream2 != nullptr;
+    hipStream_t st2 = overlap ? L.stream2 : st;
+    uint32_t* spill_sh = overlap ? B.spill2 : B.spill;
     for (int i = 0; i < n; ++i) {
         const int cur = i & 1, nxt = 1 - cur;
         const PathBuf& X = copies[cur];
@@ -1282,18 +1303,25 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
                                       L.grid_uniform, st);
           else
               launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
+        if (overlap) (void)hipStreamWaitEvent(st, L.ev_shadow, 0);
         { Bracket b(hook, "shade", st);
           hipLaunchKernelGGL(k_shade_c<BATCH>, dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl, cur,
                              nxt, B.L_out); }
-        { Bracket b(hook, "trace_shadow", st);
+        if (overlap) {
+            (void)hipEventRecord(L.ev_shade, st);
+            (void)hipStreamWaitEvent(st2, L.ev_shade, 0);
+        }
+        { Bracket b(hook, "trace_shadow", st2);
           if (one_scene)
               launch_shadow_t<false>(count, B.stack_k, plain_work(L), X, ctl, nullptr, B.L_out, B, L.refill,
-                                     L.grid_uniform, st);
+                                     L.grid_uniform, st2, cur, spill_sh);
           else
               launch_shadow_t<BATCH>(count, B.stack_k, W, X, ctl, nullptr, B.L_out, B, L.refill, L.grid_trace,
-                                     st); }
+                                     st2, cur, spill_sh); }
+        if (overlap) (void)hipEventRecord(L.ev_shadow, st2);
         if (hook) hook->after_bounce(cur);
     }
+    if (overlap) (void)hipStreamWaitEvent(st, L.ev_shadow, 0);
 }
 
 template <bool BATCH>
