@@ -505,10 +505,17 @@ struct Vertex {
     uint32_t rng;
     float4 sh_o, sh_d, nee_a, nee_b;  // shadow ray (sh_o.w t max) and NEE factors, when shadow
 };
+struct ShadeStage;
 // Lacc: the path's radiance accumulator (compact rounds), read and written
 // only when the vertex emits; null: the radiance is P.L[at], returned in v.L.
+// stage: the shadow ray and NEE factors go to LDS slot `slot` as soon as they
+// are known (they do not occupy registers through BSDF sampling); null: they
+// are returned in v.
+__device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const float4& o, const float4& d,
+                                             const float4& a, const float4& b);
 __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem& it, const PathBuf& P, uint32_t at,
-                                             int4 h, Vertex& v, float4* Lacc = nullptr) {
+                                             int4 h, Vertex& v, float4* Lacc = nullptr, ShadeStage* stage = nullptr,
+                                             int slot = 0) {
     const float4 o4 = P.o[at], d4 = P.d[at];
     const Ray r = {xyz(o4), xyz(d4)};
     const float prev_pdf = d4.w;
@@ -539,10 +546,13 @@ __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem&
         if (ls.pdf != 0.0f) {
             const BSDF e = mat_eval(m, wo_l, to_local(g, ls.wi));
             const float dist = length(ls.p - g.p);
-            v.sh_o = f4(g.p, dist - 1e-3f);
-            v.sh_d = f4(ls.wi, 0.0f);
-            v.nee_a = f4(beta * e.f, 0.0f);
-            v.nee_b = f4(ls.L * rcp_e(e.pdf + ls.pdf), 0.0f);
+            const float4 so = f4(g.p, dist - 1e-3f), sd = f4(ls.wi, 0.0f);
+            const float4 na = f4(beta * e.f, 0.0f), nb = f4(ls.L * rcp_e(e.pdf + ls.pdf), 0.0f);
+            if (stage) {
+                stage_shadow(stage, slot, so, sd, na, nb);
+            } else {
+                v.sh_o = so; v.sh_d = sd; v.nee_a = na; v.nee_b = nb;
+            }
             v.shadow = true;
         }
         const float ul = lcg(rng);
@@ -611,80 +621,95 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
     }
 }
 
-// Compact rounds: position i of the current copy X for every active path
-// (misses included: they end here).  A path that continues moves to its
-// compacted position in the other copy Y.  Radiance accumulates in L_out[path
-// id] (emission here, NEE terms in k_trace_shadow).  Shadow rays are stored
-// by their own position, with the item (batch) in sh_d.w and the path id in
-// nee_a.w.  The vertices shaded are counted in bin_count[0].
-// The vertex results wait in LDS while the block computes the output
-// positions, so they do not occupy registers across that step (the kernel
-// would otherwise need about 160 VGPRs, three waves per SIMD).
+// Vertex results of a tile of k_shade_c, staged in LDS.
 struct ShadeStage {
     float4 o[256], d[256], beta[256];
     float4 sh_o[256], sh_d[256], nee_a[256], nee_b[256];
     uint32_t rng[256];
 };
+__device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const float4& o, const float4& d,
+                                             const float4& a, const float4& b) {
+    stage->sh_o[slot] = o;
+    stage->sh_d[slot] = d;
+    stage->nee_a[slot] = a;
+    stage->nee_b[slot] = b;
+}
+// One tile of positions [base, base + 256) of k_shade_c: shading into LDS
+// (the vertex results wait there while the block computes the output
+// positions, so they do not occupy registers across that step; the kernel
+// would otherwise need about 160 VGPRs, three waves per SIMD), then the
+// output positions from `place`, then the stores.
+template <bool BATCH, typename Place>
+__device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& X, const PathState& Y, float4* L_out,
+                                           ShadeStage& sg, uint32_t base, uint32_t total, uint32_t& shaded,
+                                           Place place) {
+    const int t = (int)threadIdx.x;
+    const uint32_t i = base + threadIdx.x;
+    bool cont = false, shadow = false;
+    uint32_t pid = 0;
+    int k = 0;
+    if (i < total) {
+        pid = X.pid[i];
+        k = W.item_of(X, i);
+        const int4 h = X.hit[i];
+        Vertex v;
+        v.cont = false;
+        v.shadow = false;
+        if (h.x >= 0) {
+            ++shaded;
+            shade_vertex(W.scene(k), W.item(k), X, i, h, v, &L_out[pid], &sg, t);
+        }
+        cont = v.cont;
+        shadow = v.shadow;
+        sg.rng[t] = v.rng;
+        if (cont) {
+            sg.o[t] = v.o;
+            sg.d[t] = v.d;
+            sg.beta[t] = v.beta;
+        }
+    }
+    uint32_t pos[2];
+    place(cont, shadow, pos);
+    if (i < total) {
+        if (cont) {
+            const uint32_t j = pos[0];
+            Y.o[j] = sg.o[t];
+            Y.d[j] = sg.d[t];
+            Y.beta[j] = sg.beta[t];
+            Y.rng[j] = sg.rng[t];
+            Y.pid[j] = pid;
+            if (BATCH) Y.item[j] = k;
+        }
+        if (shadow) {
+            const uint32_t q = pos[1];
+            const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
+            X.sh_o[q] = sg.sh_o[t];
+            X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
+            X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(pid));
+            X.nee_b[q] = sg.nee_b[t];
+        }
+    }
+}
+
+// Compact rounds: position i of the current copy X for every active path
+// (misses included: they end here).  A path that continues moves to its
+// compacted position in the other copy Y.  Radiance accumulates in L_out[path
+// id] (emission here, NEE terms in k_trace_shadow).  Shadow rays are stored
+// by their own position, with the item (batch) in sh_d.w and the path id in
+// nee_a.w.  Blocks stride over the tiles and append in completion order (one
+// atomic per block and output; an order-preserving scan was measured: the
+// traversal of the next bounce ran no faster).  The vertices shaded are
+// counted in bin_count[0].
 template <bool BATCH>
 __global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
                                                     int nxt, float4* L_out) {
     __shared__ ShadeStage sg;
     const uint32_t total = ctl->q_count[cur];
-    const int t = (int)threadIdx.x;
     uint32_t shaded = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool cont = false, shadow = false;
-        uint32_t pid = 0;
-        int k = 0;
-        if (i < total) {
-            pid = X.pid[i];
-            k = W.item_of(X, i);
-            const int4 h = X.hit[i];
-            Vertex v;
-            v.cont = false;
-            v.shadow = false;
-            if (h.x >= 0) {
-                ++shaded;
-                shade_vertex(W.scene(k), W.item(k), X, i, h, v, &L_out[pid]);
-            }
-            cont = v.cont;
-            shadow = v.shadow;
-            sg.rng[t] = v.rng;
-            if (cont) {
-                sg.o[t] = v.o;
-                sg.d[t] = v.d;
-                sg.beta[t] = v.beta;
-            }
-            if (shadow) {
-                sg.sh_o[t] = v.sh_o;
-                sg.sh_d[t] = v.sh_d;
-                sg.nee_a[t] = v.nee_a;
-                sg.nee_b[t] = v.nee_b;
-            }
-        }
-        uint32_t pos[2];
-        block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->shc[cur]}, pos);
-        if (i < total) {
-            if (cont) {
-                const uint32_t j = pos[0];
-                Y.o[j] = sg.o[t];
-                Y.d[j] = sg.d[t];
-                Y.beta[j] = sg.beta[t];
-                Y.rng[j] = sg.rng[t];
-                Y.pid[j] = pid;
-                if (BATCH) Y.item[j] = k;
-            }
-            if (shadow) {
-                const uint32_t q = pos[1];
-                const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
-                X.sh_o[q] = sg.sh_o[t];
-                X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
-                X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(pid));
-                X.nee_b[q] = sg.nee_b[t];
-            }
-        }
-    }
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
+        shade_tile<BATCH>(W, X, Y, L_out, sg, base, total, shaded, [&](bool cont, bool shadow, uint32_t (&pos)[2]) {
+            block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->shc[cur]}, pos);
+        });
     for (int off = 32; off > 0; off >>= 1) shaded += __shfl_xor(shaded, off);
     if (lane_id() == 0 && shaded) atomicAdd(&ctl->bin_count[0], shaded);
 }
