@@ -84,7 +84,7 @@ struct Ctx {
     hipEvent_t ev_shade = nullptr, ev_shadow = nullptr;
     bool overlap_shadow = true;           // trace shadow rays on stream2 in plain path-tracing frames
     int trace_blocks_per_cu = 7;
-    int refill = 32;                      // idle lanes that trigger a refill
+    int refill = 40;                      // idle lanes that trigger a refill
     size_t chunk_slots = (size_t)64 << 20;       // path slots in flight
     size_t window_paths = 0;                     // paths per round; 0: as many as slots
     std::vector<BlasRec> blas;

@@ -211,7 +211,6 @@ __global__ __launch_bounds__(256) void k_regen_c(Work<BATCH> W, PathBuf X, const
             const Ray r = camera_ray(S, x, y, uPix, uLens);
             X.o[pos] = f4(r.o, __int_as_float(0));
             X.d[pos] = f4(r.d, 0.0f);
-            X.beta[pos] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             X.rng[pos] = rng;
         } else {
             X.o[pos] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
@@ -523,7 +522,8 @@ __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem&
     const Geom g = hit_geometry(S, r, h.x, h.y, __int_as_float(h.z));
     const int4 info = S.insts[h.x].info;
     V3 L = Lacc ? V3{0.0f, 0.0f, 0.0f} : xyz(P.L[at]);
-    V3 beta = xyz(P.beta[at]);
+    // a camera path starts with beta = 1 (regen_c does not store it)
+    V3 beta = depth == 0 ? V3{1.0f, 1.0f, 1.0f} : xyz(P.beta[at]);
     v.cont = false;
     v.shadow = false;
     if (info.w >= 0) {

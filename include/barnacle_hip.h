@@ -361,7 +361,7 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "window_paths"         paths per round, radiance kept by path id (default 0:
  *                          as many as chunk_slots, every path in flight at once)
  *   "trace_blocks_per_cu"  persistent blocks per CU of the trace kernels
- *   "refill"               idle lanes of a wave that trigger a refill (1..64)
+ *   "refill"               idle lanes of a wave that trigger a refill (1..64, default 40)
  *   "mlt_chunk"            PSSMLT chains (bootstrap paths) in flight per pass
  *   "stack_k"              LDS window of the traversal stack: 16 (default) or 32
  *   "graph"                1 (default): replay batched rounds as a HIP graph

@@ -34,7 +34,7 @@ def test_launch_parameters_do_not_change_film(ctx, mesh_job, opts):
         got = ds.render_pt(p)
         films, _ = ctx.render_batch([(ds, p), (ds, p)])
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 32, "chunk_slots": 64 << 20,
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 40, "chunk_slots": 64 << 20,
                      "material_sort": -1}.items():
             ctx.set_option(k, v)
     assert np.array_equal(got.view(np.int32), ref.view(np.int32))
@@ -56,6 +56,6 @@ def test_autotune(ctx, mesh_job):
         rb = ctx.autotune([(ds, p), (ds, p)], batched=True)
         assert rb["trials"] >= 8
     finally:
-        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 32, "chunk_slots": 64 << 20,
+        for k, v in {"stack_k": 16, "trace_blocks_per_cu": 7, "refill": 40, "chunk_slots": 64 << 20,
                      "material_sort": -1}.items():
             ctx.set_option(k, v)
