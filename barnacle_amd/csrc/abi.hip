@@ -824,7 +824,13 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         hip_check(hipMemcpyAsync(c->b_hdr.p, &h, sizeof(h), hipMemcpyHostToDevice, s), "round header");
         L.scene = scenes[k0];
         scene_rounds += L.uniform;
-        const hipGraphExec_t exec = block_graph(L);
+        // A large round over one scene runs eagerly with its shadow rays on the
+        // second stream (as a plain frame does); other rounds replay a graph.
+        const bool overlap = c->overlap_shadow && L.uniform && path >= (1 << 20);
+        L.stream2 = overlap ? c->stream2 : nullptr;
+        L.ev_shade = c->ev_shade;
+        L.ev_shadow = c->ev_shadow;
+        const hipGraphExec_t exec = overlap ? nullptr : block_graph(L);
         used_graph = used_graph || exec;
         run_round(c, true, bnd::Integrator::Path, L, P, ctl, B, (uint32_t)path, round_depth, false, &hook, exec);
         hip_check(hipStreamSynchronize(s), "round");
