@@ -201,6 +201,50 @@ def bvh_builds(ctx, scs):
             "host": "one thread (the reference's build is sequential too)"}
 
 
+def counter_summary(out_path, dbs, commands):
+    """Per-kernel averages per dispatch of the counters in rocprofv3 result
+    databases (one counter pass each; counters_collection view) of the config-1
+    frame, written as the JSON that roofline() reads: FETCH_SIZE / WRITE_SIZE
+    in KiB with the calibration of the run's largest device copy (a 1 GiB
+    hipMemcpy reads and writes exactly 2^30 bytes), TCC_HIT_sum / TCC_MISS_sum,
+    and any SQ counters.  Kernel names drop their parameter lists."""
+    import re
+    import sqlite3
+    kern = {}
+    copy = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        per = {}
+        for kname, cname, val, did, dur in con.execute(
+                "select kernel_name, counter_name, value, dispatch_id, duration from counters_collection"):
+            short = re.sub(r"\(.*", "", kname).replace("void ", "").replace("bnd::", "")
+            d = per.setdefault((short, cname), {})
+            d[did] = d.get(did, 0.0) + val                    # summed over dimensions (XCDs, channels)
+            if cname in ("FETCH_SIZE", "WRITE_SIZE") and "copyBuffer" in short:
+                copy[cname] = max(copy.get(cname, 0.0), d[did])
+            kern.setdefault(short, {}).setdefault("_dur", {})[did] = dur
+        for (short, cname), d in per.items():
+            k = kern[short]
+            k[cname] = sum(d.values()) / len(d)
+            k["dispatches"] = max(k.get("dispatches", 0), len(d))
+    out = {"workload": "config1 1024x1024x64spp: one counting frame (<true, ...> traversal variants) and one plain "
+                       "frame", "commands": commands, "kernels": {}, "copy_calibration_bytes": 1 << 30,
+           "copy_kib": copy}
+    for short, k in sorted(kern.items()):
+        durs = k.pop("_dur")
+        k["avg_ms_under_counters"] = sum(durs.values()) / len(durs) / 1e6
+        if "TCC_HIT_sum" in k and "TCC_MISS_sum" in k:
+            k["l2_hit_rate"] = k["TCC_HIT_sum"] / max(k["TCC_HIT_sum"] + k["TCC_MISS_sum"], 1.0)
+        if "FETCH_SIZE" in k and "WRITE_SIZE" in k and copy.get("FETCH_SIZE") and copy.get("WRITE_SIZE"):
+            ff, wf = copy["FETCH_SIZE"] * 1024 / (1 << 30), copy["WRITE_SIZE"] * 1024 / (1 << 30)
+            k["hbm_bytes_per_launch"] = k["FETCH_SIZE"] * 1024 / ff + k["WRITE_SIZE"] * 1024 / wf
+            out["fetch_factor"], out["write_factor"] = ff, wf
+        out["kernels"][short] = k
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
+
+
 def timeline_summary(csv_path, min_phase_gap_s=0.2):
     """Idle time of the GPU in a rocprofv3 kernel trace (kernel_trace.csv) of
     `bench.py --batch-timeline`: the trace splits into phases at gaps of at
@@ -447,13 +491,24 @@ def main():
     ap.add_argument("--no-pssmlt", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="trace shadow rays after shading on the one stream in every frame")
     ap.add_argument("--batch-timeline", action="store_true",
                     help="run only the config-2 plain/batched leg, with pauses between its phases")
     ap.add_argument("--timeline-summary", metavar="KERNEL_TRACE_CSV",
                     help="summarise a rocprofv3 kernel trace of --batch-timeline (no GPU use)")
+    ap.add_argument("--counter-summary", nargs="+", metavar=("OUT_JSON", "RESULTS_DB"),
+                    help="merge rocprofv3 counter databases of the config-1 frame into OUT_JSON (no GPU use)")
+    ap.add_argument("--command", action="append", default=[],
+                    help="with --counter-summary: a command that made the databases (recorded)")
     args = ap.parse_args()
     if args.timeline_summary:
         print(json.dumps(timeline_summary(args.timeline_summary), indent=1))
+        return
+    if args.counter_summary:
+        r = counter_summary(args.counter_summary[0], args.counter_summary[1:], args.command)
+        print(json.dumps({k: {c: v for c, v in d.items() if not isinstance(v, dict)} for k, d in r["kernels"].items()},
+                         indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the renderer has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -469,6 +524,8 @@ def main():
     params = sc.pt_params()
     ctx = native.Context(0)
     ds = ctx.scene(sc)
+    if args.no_overlap:
+        ctx.set_option("overlap_shadow", 0)
     print(f"# scene: {sc.n_triangles} triangles, {params.width}x{params.height}x{params.spp}spp depth "
           f"{params.max_depth}", file=sys.stderr, flush=True)
 
@@ -479,16 +536,23 @@ def main():
 
     for _ in range(args.warmup):
         ds.render_pt(params)
-    ctx.set_option("timing", 1)
-    ctx.reset_kernel_times()
     t0 = time.perf_counter()
     film = None
     for i in range(args.steps):
         film = ds.render_pt(params)
         print(f"# step {i} done", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
+    # Per-kernel launch times for the rooflines: one more frame with HIP events
+    # around every launch and the kernels one after another (the timed frames
+    # trace each iteration's shadow rays beside the next closest-hit traversal,
+    # so a kernel's own duration is not defined there).
+    ctx.set_option("overlap_shadow", 0)
+    ctx.set_option("timing", 1)
+    ctx.reset_kernel_times()
+    ds.render_pt(params)
     kt = ctx.kernel_times()
     ctx.set_option("timing", 0)
+    ctx.set_option("overlap_shadow", 0 if args.no_overlap else 1)
 
     ms_per_step = elapsed / args.steps * 1e3
 
@@ -500,16 +564,16 @@ def main():
             return {}
         with open(path) as fh:
             return json.load(fh)
-    pmc = load_profile("trace_closest_config1_pmc.json")
-    l2 = load_profile("l2_hit_config1.json").get("kernels", {})
-    traffic_of = {"trace_closest": pmc.get("hbm_bytes_per_launch"),
-                  "trace_shadow": pmc.get("trace_shadow", {}).get("hbm_bytes_per_launch"),
-                  "shade_c": pmc.get("shade", {}).get("hbm_bytes_per_launch")}
+    prof = load_profile("counters_config1.json").get("kernels", {})
+    names = {"trace_closest": "k_trace_closest<false, false, 16>", "trace_shadow": "k_trace_shadow<false, false, 16>",
+             "shade_c": "k_shade_c<false>"}
+    traffic_of = {k: prof.get(v, {}).get("hbm_bytes_per_launch") for k, v in names.items()}
+    l2 = {k: {"l2_hit_rate": prof.get(v, {}).get("l2_hit_rate")} for k, v in names.items()}
 
     def roofline(kernel, timer_name, frame_algorithmic):
         launches, total_ms = kt[timer_name]
         per_launch_ms = total_ms / launches
-        bytes_per_launch = frame_algorithmic / (launches / args.steps)
+        bytes_per_launch = frame_algorithmic / launches        # one frame
         achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
         traffic = traffic_of.get(kernel)
         r = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -522,10 +586,10 @@ def main():
             r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
         return r
     rl_closest = roofline("trace_closest", "trace_closest", closest_bytes(st))
-    # The traversal kernels keep 85% (trace_closest) of their VALU issue slots
-    # busy (profiles/trace_config1_sq_counters.md): the byte roofline is the
-    # contract's yardstick, instruction issue is what limits them.
-    rl_closest["limiter"] = "VALU instruction issue (85% of issue slots, profiles/trace_config1_sq_counters.md)"
+    # The closest-hit traversal keeps 87% of its VALU issue slots busy
+    # (profiles/trace_config1_sq_counters.md): the byte roofline is the
+    # contract's yardstick, instruction issue is what limits it.
+    rl_closest["limiter"] = "VALU instruction issue (87% of issue slots, profiles/trace_config1_sq_counters.md)"
 
     out = {
         "metric": "Mrays/s (batched vs plain, %HBM peak) + adjoint iters/s at RMSE<=1e-3 vs CPU",
@@ -550,7 +614,9 @@ def main():
         "roofline": rl_closest,
         "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st)),
                       "shade_c": roofline("shade_c", "shade", shade_bytes(st))},
-        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in kt.items()},
+        "kernel_ms_per_step": {k: v[1] for k, v in kt.items()},
+        "kernel_ms_note": "one frame with the kernels one after another and HIP events around each launch",
+        "overlap_shadow": not args.no_overlap,
     }
     if not args.no_batch:
         out["batched"] = batched(ctx)
