@@ -90,3 +90,28 @@ def test_pssmlt_odd_counts_match_oracle(ctx, orc, n_bootstrap, n_chains):
     assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
     assert gst["b"] == ost["b"] and gst["mutations_per_chain"] == ost["mutations_per_chain"]
     assert np.array_equal(gacc, oacc)
+
+
+def test_pssmlt_config4_geometry_matches_oracle(ctx, orc):
+    """Config 4 at its real geometry (two 20,480-triangle spheres, dielectric
+    and GGX metal, depth 16, large step 0.3, sigma 1/64) with a reduced film
+    and chain count: bootstrap luminances, B and per-chain accept counts
+    bit-exact, films within rel-RMSE 1e-5 with equal NaN masks."""
+    info = configs.caustic_scene(width=48, height=48, spp=16, max_depth=16, freq=32,
+                                 n_bootstrap=1 << 14, n_chains=1 << 11)
+    sc = scene.load_scene(info)
+    assert sc.n_triangles > 40000
+    p = sc.mlt_params()
+    ds = ctx.scene(sc)
+    gfilm, gst = ds.render_pssmlt(p)
+    gboot, gacc = ds.pssmlt_last(p)
+    ofilm, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
+    assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
+    assert gst["b"] == ost["b"] and gst["b"] > 0
+    assert np.array_equal(gacc, oacc)
+    assert gst["accepted"] == ost["accepted"] and gst["proposed"] == ost["proposed"]
+    gnan, onan = ~np.isfinite(gfilm), ~np.isfinite(ofilm)
+    assert np.array_equal(gnan, onan)
+    ok = ~gnan.any(axis=1)
+    g, o = gfilm[ok], ofilm[ok]
+    assert np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)) <= 1e-5
