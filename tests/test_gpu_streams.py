@@ -1,0 +1,69 @@
+"""The two-stream schedule of compact path-tracing rounds: the shadow rays of
+iteration i traced on a second stream beside iteration i + 1's closest-hit
+traversal.  Each path adds its radiance terms in the reference's order
+whatever the overlap, so films are bit-identical with the overlap on and off,
+and so are the exact work counters of a counting render.  Batches run their
+large rounds over one scene with the same schedule."""
+import numpy as np
+import pytest
+
+from barnacle_amd import configs, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def with_overlap(ctx, on, fn):
+    ctx.set_option("overlap_shadow", 1 if on else 0)
+    try:
+        return fn()
+    finally:
+        ctx.set_option("overlap_shadow", 1)
+
+
+@pytest.mark.parametrize("name", ["config0", "config1", "spheres", "batch7"])
+def test_overlap_keeps_films_and_counters(ctx, name):
+    info = {"config0": lambda: configs.config0(),
+            "config1": lambda: configs.config1(width=160, height=120, spp=8, freq=24),
+            "spheres": lambda: configs.sphere_scene(96, 64, 8, 8),
+            "batch7": lambda: configs.batch_scene(7, max_res=128, max_spp=8)}[name]()
+    sc = scene.load_scene(info)
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    print(f"overlap on/off: {name}", flush=True)
+    a, sa = with_overlap(ctx, False, lambda: ds.render_pt(p, stats=True))
+    b, sb = with_overlap(ctx, True, lambda: ds.render_pt(p, stats=True))
+    c = with_overlap(ctx, True, lambda: ds.render_pt(p))
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    assert np.array_equal(a.view(np.int32), c.view(np.int32))
+    assert sa == sb
+    assert sa["rays_shadow"] > 0
+
+
+def test_overlap_over_several_rounds(ctx):
+    """A frame split into rounds (window smaller than the frame's paths):
+    every round joins its streams before the next begins."""
+    sc = scene.load_scene(configs.config1(width=128, height=128, spp=6, freq=16))
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    ref = with_overlap(ctx, False, lambda: ds.render_pt(p))
+    ctx.set_option("window_paths", 40000)
+    try:
+        got = with_overlap(ctx, True, lambda: ds.render_pt(p))
+    finally:
+        ctx.set_option("window_paths", 0)
+    assert np.array_equal(ref.view(np.int32), got.view(np.int32))
+
+
+def test_batch_round_over_one_scene_uses_two_streams(ctx):
+    """An item of 2^20 paths or more gets a round over its own scene, run
+    eagerly with the two-stream schedule; the small item of another scene
+    gets a graph-replayed round.  Both equal their plain renders."""
+    big_sc = scene.load_scene(configs.config1(width=256, height=256, spp=16, freq=16))
+    small_sc = scene.load_scene(configs.config0())
+    big = (ctx.scene(big_sc), big_sc.pt_params())
+    small = (ctx.scene(small_sc), small_sc.pt_params(width=32, height=32, spp=2))
+    assert big[1].width * big[1].height * big[1].spp >= 1 << 20
+    films, st = ctx.render_batch([big, small])
+    assert st["scene_rounds"] >= 1
+    for (ds, p), f in zip([big, small], films):
+        assert np.array_equal(f.view(np.int32), with_overlap(ctx, False, lambda: ds.render_pt(p)).view(np.int32))
