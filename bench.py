@@ -78,8 +78,19 @@ def host_cpu():
                     break
     except OSError:
         pass
+    lscpu = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)"):
+                lscpu[k.strip()] = v.strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
     # One GPU's share of the GPU box's host is 16 threads; use all of them.
-    return {"threads": min(16, avail), "available_threads": avail, "cpu_count": os.cpu_count(), "model": model}
+    return {"threads": min(16, avail), "available_threads": avail, "cpu_count": os.cpu_count(), "model": model,
+            "lscpu": lscpu}
 
 
 def cpu_baseline(sc, params, gpu_film=None, budget_s=12.0):
