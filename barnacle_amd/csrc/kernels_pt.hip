@@ -701,7 +701,7 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
 // traversal of the next bounce ran no faster).  The vertices shaded are
 // counted in bin_count[0].
 template <bool BATCH>
-__global__ __launch_bounds__(256, 4) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
+__global__ __launch_bounds__(256, 3) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
                                                     int nxt, float4* L_out) {
     __shared__ ShadeStage sg;
     const uint32_t total = ctl->q_count[cur];
@@ -1299,12 +1299,15 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
     // The trace kernels read only the geometry of a ray's scene, so a batch
     // round over one scene traces with the single-scene kernels.
     const bool one_scene = BATCH && L.uniform;
-    // regen_c and shade_c run one resident set of blocks (four 256-thread
-    // blocks per CU, shade_c's launch bound), each striding over the tiles
-    // of positions.  With twice as many blocks the traversal of the next
-    // bounce, which reads the positions in the order the blocks appended
-    // them, was measured 1.6% (config 1) to 3.6% (32-scene batch) slower.
-    const int grid_c = L.grid_shade / 2;
+    // regen_c and shade_c run one resident set of blocks (three 256-thread
+    // blocks per CU, shade_c's launch bound: 139 VGPRs, no spills), each
+    // striding over the tiles of positions.  With twice as many blocks the
+    // traversal of the next bounce, which reads the positions in the order
+    // the blocks appended them, was measured 1.6% (config 1) to 3.6%
+    // (32-scene batch) slower; four blocks per CU at four waves per SIMD
+    // (6 VGPRs spilled) made shade_c 0.7 ms faster per config-1 frame but
+    // the two-stream frame 2.4 ms slower.
+    const int grid_c = L.grid_shade * 3 / 8;
     // With a second stream, the shadow rays of iteration i are traced while
     // regen, prep and trace_closest of iteration i + 1 run: shade_c(i + 1)
     // waits for trace_shadow(i) (it reuses the shadow arrays and adds to the
