@@ -444,15 +444,26 @@ void run_round(Ctx* c, bool batch, bnd::Integrator kind, const bnd::Launch& L, c
     // every path needs at most max_depth + 1 iterations once it has a slot
     const int64_t limit = ((int64_t)n_paths / L.n_slots + 2) * (int64_t)(max_depth + 2) * 2 + 4 * kIterBlock;
     int64_t iters = 0;
+    // A round whose paths all fit its slots issues every path in its first
+    // iteration, and a path retires within max_depth iterations of its
+    // issue: the first check waits until then (the iterations in between run
+    // back to back, with no host synchronisation between blocks).
+    const bool all_at_once = n_paths <= L.n_slots;
     for (;;) {
+        int n = kIterBlock;
+        if (iters == 0 && all_at_once)
+            n = std::max(kIterBlock, block ? (max_depth + kIterBlock - 1) / kIterBlock * kIterBlock
+                                           : (max_depth + 1) / 2 * 2);
         if (block) {
-            c->timer.begin("iteration_block", s);
-            hip_check(hipGraphLaunch(block, s), "graph launch");
-            c->timer.end(s);
+            for (int r = 0; r < n / kIterBlock; ++r) {
+                c->timer.begin("iteration_block", s);
+                hip_check(hipGraphLaunch(block, s), "graph launch");
+                c->timer.end(s);
+            }
         } else {
-            bnd::enqueue_iterations(batch, kind, L, P, ctl, B, kIterBlock, count, hook);
+            bnd::enqueue_iterations(batch, kind, L, P, ctl, B, n, count, hook);
         }
-        iters += kIterBlock;
+        iters += n;
         bnd::Ctl h;
         hip_check(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "ctl readback");
         hip_check(hipStreamSynchronize(s), "iteration block");
