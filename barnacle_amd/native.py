@@ -241,9 +241,17 @@ class DeviceScene:
         check(self.ctx.lib.bn_scene_export_instances(self.ctx.h, self.id, out, C.byref(n)))
         return [out[i] for i in range(n.value)]
 
-    def render_pt(self, params: abi.PTParams, stats: bool = False):
-        """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters."""
-        film = np.zeros((params.width * params.height, 3), np.float32)
+    def render_pt(self, params: abi.PTParams, stats: bool = False, out=None):
+        """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters.
+        out: a C-contiguous float32 (H*W, 3) array to render into (as the
+        reference renders into its existing Film.Pixels)."""
+        n = params.width * params.height
+        if out is None:
+            film = np.zeros((n, 3), np.float32)
+        else:
+            if out.dtype != np.float32 or out.shape != (n, 3) or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float32 array of shape (width*height, 3)")
+            film = out
         st = abi.TraceStats() if stats else None
         check(self.ctx.lib.bn_render_pt(self.ctx.h, self.id, C.byref(params), _p(film, f32p),
                                         C.byref(st) if st is not None else None))
