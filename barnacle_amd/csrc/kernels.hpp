@@ -108,6 +108,9 @@ struct Launch {
     // for rounds captured into a graph.
     hipStream_t stream2;
     hipEvent_t ev_shade, ev_shadow;
+    // Compact shading groups each block's continuing paths and shadow rays
+    // by the octant of their ray direction.
+    bool octant_order;
 };
 
 // Path-replay adjoint state (section "adjoint" of kernels_pt.hip).

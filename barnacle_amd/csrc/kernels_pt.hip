@@ -480,6 +480,57 @@ __device__ __forceinline__ void block_pos(const bool (&pred)[N], uint32_t* const
     __syncthreads();
 }
 
+// Block-aggregated positions in two queues, each block's entries grouped by
+// a 3-bit key (thread order within a key).  For queue o: rank[o] is the
+// thread's rank among the block's entries when p[o] holds, base[o] the
+// position of the block's first entry and n[o] the block's entry count.
+// Four ballots per queue and wave give the per-key masks; one wave scans the
+// 2 x 8 keys x 4 waves counts.  Every thread of the 256-thread block must call
+// it; the caller synchronises the block before the next call.
+__device__ __forceinline__ void block_rank_oct(const bool (&p)[2], const int (&key)[2], uint32_t* const (&ctr)[2],
+                                               uint32_t (&rank)[2], uint32_t (&base)[2], uint32_t (&n)[2]) {
+    constexpr int NW = 4;
+    __shared__ uint32_t cnt[2][8][NW];
+    __shared__ uint32_t tot[2][2];
+    const int w = (int)(threadIdx.x >> 6), l = (int)lane_id();
+    uint64_t mk[2];
+    for (int o = 0; o < 2; ++o) {
+        const uint64_t mv = __ballot(p[o]);
+        const uint64_t m0 = __ballot(key[o] & 1), m1 = __ballot(key[o] & 2), m2 = __ballot(key[o] & 4);
+        auto of_key = [&](int b) {
+            return mv & (b & 1 ? m0 : ~m0) & (b & 2 ? m1 : ~m1) & (b & 4 ? m2 : ~m2);
+        };
+        mk[o] = of_key(key[o]);
+        if (l < 8) cnt[o][l][w] = (uint32_t)__popcll(of_key(l));
+    }
+    __syncthreads();
+    if (w == 0) {
+        // lanes [0, 32): queue 0, [32, 64): queue 1; entry j = key * 4 + wave
+        const int o = l >> 5, j = l & 31;
+        const uint32_t c = cnt[o][j >> 2][j & 3];
+        uint32_t incl = c;
+        for (int off = 1; off < 32; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off);
+            if (j >= off) incl += u;
+        }
+        const uint32_t total = __shfl(incl, o * 32 + 31);
+        uint32_t b0 = 0;
+        if (j == 0 && total) b0 = atomicAdd(ctr[o], total);
+        b0 = __shfl(b0, o * 32);
+        cnt[o][j >> 2][j & 3] = incl - c;
+        if (j == 0) {
+            tot[o][0] = total;
+            tot[o][1] = b0;
+        }
+    }
+    __syncthreads();
+    for (int o = 0; o < 2; ++o) {
+        rank[o] = cnt[o][key[o]][w] + mbcnt(mk[o]);
+        n[o] = tot[o][0];
+        base[o] = tot[o][1];
+    }
+}
+
 // Block-aggregated append of `val` to N queues.
 template <int N, int NW = 4>
 __device__ __forceinline__ void block_push(const bool (&pred)[N], uint32_t* const (&ctr)[N],
@@ -625,7 +676,9 @@ __global__ __launch_bounds__(256) void k_shade(Work<BATCH> W, PathBuf P, Ctl* ct
 struct ShadeStage {
     float4 o[256], d[256], beta[256];
     float4 sh_o[256], sh_d[256], nee_a[256], nee_b[256];
-    uint32_t rng[256];
+    uint32_t rng[256], pid[256];
+    int item[256];
+    uint16_t inv[2][256];
 };
 __device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const float4& o, const float4& d,
                                              const float4& a, const float4& b) {
@@ -634,15 +687,25 @@ __device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const 
     stage->nee_a[slot] = a;
     stage->nee_b[slot] = b;
 }
+// Octant of a ray direction (sign bits of x, y, z).
+__device__ __forceinline__ int octant(float4 d) {
+    return (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
+}
 // One tile of positions [base, base + 256) of k_shade_c: shading into LDS
 // (the vertex results wait there while the block computes the output
 // positions, so they do not occupy registers across that step; the kernel
 // would otherwise need about 160 VGPRs, three waves per SIMD), then the
-// output positions from `place`, then the stores.
-template <bool BATCH, typename Place>
+// output positions, then the stores.
+//   ORDER: the block's continuing paths and its shadow rays are each grouped
+//   by the octant of their ray direction, so that the waves of the next
+//   traversal, which take consecutive positions, hold rays of one octant
+//   (same near-child order at every node); the stores then read the stage
+//   in that order and write consecutive positions.  A path's results do not
+//   depend on its position, so the film is the same either way.
+template <bool BATCH, bool ORDER>
 __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& X, const PathState& Y, float4* L_out,
                                            ShadeStage& sg, uint32_t base, uint32_t total, uint32_t& shaded,
-                                           Place place) {
+                                           uint32_t* const (&ctr)[2]) {
     const int t = (int)threadIdx.x;
     const uint32_t i = base + threadIdx.x;
     bool cont = false, shadow = false;
@@ -668,27 +731,58 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
             sg.beta[t] = v.beta;
         }
     }
-    uint32_t pos[2];
-    place(cont, shadow, pos);
-    if (i < total) {
-        if (cont) {
-            const uint32_t j = pos[0];
-            Y.o[j] = sg.o[t];
-            Y.d[j] = sg.d[t];
-            Y.beta[j] = sg.beta[t];
-            Y.rng[j] = sg.rng[t];
-            Y.pid[j] = pid;
-            if (BATCH) Y.item[j] = k;
+    if (!ORDER) {
+        uint32_t pos[2];
+        block_pos<2>({cont, shadow}, ctr, pos);
+        if (i < total) {
+            if (cont) {
+                const uint32_t j = pos[0];
+                Y.o[j] = sg.o[t];
+                Y.d[j] = sg.d[t];
+                Y.beta[j] = sg.beta[t];
+                Y.rng[j] = sg.rng[t];
+                Y.pid[j] = pid;
+                if (BATCH) Y.item[j] = k;
+            }
+            if (shadow) {
+                const uint32_t q = pos[1];
+                const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
+                X.sh_o[q] = sg.sh_o[t];
+                X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
+                X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(pid));
+                X.nee_b[q] = sg.nee_b[t];
+            }
         }
-        if (shadow) {
-            const uint32_t q = pos[1];
-            const float4 sd = sg.sh_d[t], na = sg.nee_a[t];
-            X.sh_o[q] = sg.sh_o[t];
-            X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(k));
-            X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(pid));
-            X.nee_b[q] = sg.nee_b[t];
-        }
+        return;
     }
+    const int key[2] = {cont ? octant(sg.d[t]) : 0, shadow ? octant(sg.sh_d[t]) : 0};
+    uint32_t rank[2], bpos[2], n[2];
+    block_rank_oct({cont, shadow}, key, ctr, rank, bpos, n);
+    sg.pid[t] = pid;
+    if (BATCH) sg.item[t] = k;
+    if (cont) sg.inv[0][rank[0]] = (uint16_t)t;
+    if (shadow) sg.inv[1][rank[1]] = (uint16_t)t;
+    __syncthreads();
+    if ((uint32_t)t < n[0]) {
+        const int u = sg.inv[0][t];
+        const uint32_t j = bpos[0] + t;
+        Y.o[j] = sg.o[u];
+        Y.d[j] = sg.d[u];
+        Y.beta[j] = sg.beta[u];
+        Y.rng[j] = sg.rng[u];
+        Y.pid[j] = sg.pid[u];
+        if (BATCH) Y.item[j] = sg.item[u];
+    }
+    if ((uint32_t)t < n[1]) {
+        const int u = sg.inv[1][t];
+        const uint32_t q = bpos[1] + t;
+        const float4 sd = sg.sh_d[u], na = sg.nee_a[u];
+        X.sh_o[q] = sg.sh_o[u];
+        X.sh_d[q] = make_float4(sd.x, sd.y, sd.z, __int_as_float(BATCH ? sg.item[u] : 0));
+        X.nee_a[q] = make_float4(na.x, na.y, na.z, __uint_as_float(sg.pid[u]));
+        X.nee_b[q] = sg.nee_b[u];
+    }
+    __syncthreads();
 }
 
 // Compact rounds: position i of the current copy X for every active path
@@ -700,16 +794,15 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
 // atomic per block and output; an order-preserving scan was measured: the
 // traversal of the next bounce ran no faster).  The vertices shaded are
 // counted in bin_count[0].
-template <bool BATCH>
+template <bool BATCH, bool ORDER>
 __global__ __launch_bounds__(256, 3) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
                                                     int nxt, float4* L_out) {
     __shared__ ShadeStage sg;
     const uint32_t total = ctl->q_count[cur];
     uint32_t shaded = 0;
+    uint32_t* const ctr[2] = {&ctl->q_count[nxt], &ctl->shc[cur]};
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
-        shade_tile<BATCH>(W, X, Y, L_out, sg, base, total, shaded, [&](bool cont, bool shadow, uint32_t (&pos)[2]) {
-            block_pos<2>({cont, shadow}, {&ctl->q_count[nxt], &ctl->shc[cur]}, pos);
-        });
+        shade_tile<BATCH, ORDER>(W, X, Y, L_out, sg, base, total, shaded, ctr);
     for (int off = 32; off > 0; off >>= 1) shaded += __shfl_xor(shaded, off);
     if (lane_id() == 0 && shaded) atomicAdd(&ctl->bin_count[0], shaded);
 }
@@ -1333,8 +1426,12 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
               launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
         if (overlap) (void)hipStreamWaitEvent(st, L.ev_shadow, 0);
         { Bracket b(hook, "shade", st);
-          hipLaunchKernelGGL(k_shade_c<BATCH>, dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl, cur,
-                             nxt, B.L_out); }
+          if (L.octant_order)
+              hipLaunchKernelGGL((k_shade_c<BATCH, true>), dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl,
+                                 cur, nxt, B.L_out);
+          else
+              hipLaunchKernelGGL((k_shade_c<BATCH, false>), dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl,
+                                 cur, nxt, B.L_out); }
         if (overlap) {
             (void)hipEventRecord(L.ev_shade, st);
             (void)hipStreamWaitEvent(st2, L.ev_shade, 0);

@@ -372,7 +372,16 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *                          -1 (default): bins for gradient rounds only, whose
  *                          atomics gain from material-coherent waves
  *   "bvh_gpu"              1 (default): bn_scene_build builds mesh BLAS on the
- *                          GPU (bn_bvh_build_gpu); 0: on the host */
+ *                          GPU (bn_bvh_build_gpu); 0: on the host
+ *   "bvh_small"            largest subtree (triangles, 128..512, default 512)
+ *                          that one workgroup of the GPU build finishes alone
+ *   "overlap_shadow"       1 (default): path-tracing frames trace each bounce's
+ *                          shadow rays on a second stream beside the next
+ *                          bounce's closest-hit traversal; 0: one stream
+ *   "octant_order"         1 (default): shading writes each workgroup's
+ *                          continuing paths and shadow rays grouped by the
+ *                          octant of their ray direction; 0: in thread order.
+ *                          Films and counters are the same either way. */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {

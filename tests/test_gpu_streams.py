@@ -3,7 +3,9 @@ iteration i traced on a second stream beside iteration i + 1's closest-hit
 traversal.  Each path adds its radiance terms in the reference's order
 whatever the overlap, so films are bit-identical with the overlap on and off,
 and so are the exact work counters of a counting render.  Batches run their
-large rounds over one scene with the same schedule."""
+large rounds over one scene with the same schedule.  The same holds for the
+order in which shading writes a workgroup's outputs (grouped by ray-direction
+octant or in thread order)."""
 import numpy as np
 import pytest
 
@@ -67,3 +69,50 @@ def test_batch_round_over_one_scene_uses_two_streams(ctx):
     assert st["scene_rounds"] >= 1
     for (ds, p), f in zip([big, small], films):
         assert np.array_equal(f.view(np.int32), with_overlap(ctx, False, lambda: ds.render_pt(p)).view(np.int32))
+
+
+@pytest.mark.parametrize("name", ["config0", "config1", "spheres", "batch7"])
+def test_octant_order_keeps_films_and_counters(ctx, name):
+    """Shading may write a workgroup's continuing paths and shadow rays in
+    any order (here grouped by ray-direction octant): every path adds its
+    terms in its own order, so films and exact counters do not change."""
+    info = {"config0": lambda: configs.config0(),
+            "config1": lambda: configs.config1(width=160, height=120, spp=8, freq=24),
+            "spheres": lambda: configs.sphere_scene(96, 64, 8, 8),
+            "batch7": lambda: configs.batch_scene(7, max_res=128, max_spp=8)}[name]()
+    sc = scene.load_scene(info)
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    print(f"octant order on/off: {name}", flush=True)
+    try:
+        ctx.set_option("octant_order", 0)
+        a, sa = ds.render_pt(p, stats=True)
+        ctx.set_option("octant_order", 1)
+        b, sb = ds.render_pt(p, stats=True)
+        c = with_overlap(ctx, False, lambda: ds.render_pt(p))
+    finally:
+        ctx.set_option("octant_order", 1)
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    assert np.array_equal(a.view(np.int32), c.view(np.int32))
+    assert sa == sb
+
+
+def test_octant_order_in_batches(ctx):
+    """Batched rounds (graph-replayed, mixed scenes) with and without the
+    octant grouping give the plain renders' films."""
+    infos = [configs.batch_scene(s, max_res=64, max_spp=4, tri_range=(300, 20000)) for s in (3, 11, 19)]
+    items = []
+    for info in infos:
+        sc = scene.load_scene(info)
+        items.append((ctx.scene(sc), sc.pt_params()))
+    try:
+        ctx.set_option("octant_order", 0)
+        ref = [ds.render_pt(p) for ds, p in items]
+        got0, _ = ctx.render_batch(items)
+        ctx.set_option("octant_order", 1)
+        got1, _ = ctx.render_batch(items)
+    finally:
+        ctx.set_option("octant_order", 1)
+    for r, a, b in zip(ref, got0, got1):
+        assert np.array_equal(r.view(np.int32), a.view(np.int32))
+        assert np.array_equal(r.view(np.int32), b.view(np.int32))
