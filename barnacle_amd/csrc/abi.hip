@@ -997,7 +997,15 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
     bnd::MltParamsDev dp{p.width, p.height, p.frame_id, p.n_bootstrap, p.large_step_prob, 0.0f, 0.0f};
     const uint32_t bin_cap = (uint32_t)c->path_cap;
 
-    // One MLT path per slot [0, n): start, bounce loop.
+    // One MLT path per slot [0, n): start, bounce loop.  With a second
+    // stream the shadow rays of bounce d are traced on it while bounce d + 1
+    // resets, traces and bins on the first; shade(d + 1) waits for them (it
+    // adds to the same radiance and reuses the shadow arrays), so every path
+    // adds its terms in the reference's order.  Shadow counts and any-hit
+    // fetch counters are kept per bounce parity, the any-hit traversal spills
+    // to its own area, and the streams are joined after the last bounce.
+    const bool overlap = c->overlap_shadow;
+    hipStream_t s2 = overlap ? c->stream2 : s;
     auto run_paths = [&](bool fresh, int id0, int n) {
         bnd::launch_ctl_clear(ctl, s);
         { Timed tm(T, "mlt_start", s);
@@ -1005,22 +1013,30 @@ void render_pssmlt(Ctx* c, SceneRec& sc, const bn_mlt_params& p, float* film_hos
         int cur = 0;
         for (int depth = 0; depth < p.max_depth; ++depth) {
             const int nxt = 1 - cur;
-            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, BF.fetch_seg, s); }
+            const int par = overlap ? (depth & 1) : -1;
+            { Timed tm(T, "reset", s); bnd::launch_reset(ctl, nxt, BF.fetch_seg, s, par); }
             { Timed tm(T, "trace_closest", s);
               bnd::launch_trace_closest(false, sc.dev, P, ctl, cur, (const uint32_t*)c->queue[cur].p, BF,
                                         (uint32_t)c->refill, gt, s); }
             { Timed tm(T, "bin", s);
               bnd::launch_bin(P, ctl, cur, (const uint32_t*)c->queue[cur].p, (uint32_t*)c->bins.p, bin_cap, nullptr,
                               c->material_sort > 0, gs, s); }
+            if (overlap) hip_check(hipStreamWaitEvent(s, c->ev_shadow, 0), "wait shadow");
             { Timed tm(T, "shade_mlt", s);
               bnd::launch_shade_mlt(fresh, sc.dev, P, M, ctl, (const uint32_t*)c->bins.p, bin_cap,
                                     (uint32_t*)c->queue[nxt].p, nxt, (uint32_t*)c->shq.p,
-                                    depth, p.max_depth, p.rr_depth, gs, s); }
-            { Timed tm(T, "trace_shadow", s);
+                                    depth, p.max_depth, p.rr_depth, gs, s, par); }
+            if (overlap) {
+                hip_check(hipEventRecord(c->ev_shade, s), "record shade");
+                hip_check(hipStreamWaitEvent(s2, c->ev_shade, 0), "wait shade");
+            }
+            { Timed tm(T, "trace_shadow", s2);
               bnd::launch_trace_shadow(false, sc.dev, P, ctl, (const uint32_t*)c->shq.p, BF, (uint32_t)c->refill,
-                                       gt, s); }
+                                       gt, s2, par, overlap ? BF.spill2 : nullptr); }
+            if (overlap) hip_check(hipEventRecord(c->ev_shadow, s2), "record shadow");
             cur = nxt;
         }
+        if (overlap) hip_check(hipStreamWaitEvent(s, c->ev_shadow, 0), "join shadow");
     };
 
     // PSSMLT.fs:382-392: bootstrap

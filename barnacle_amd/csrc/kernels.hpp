@@ -178,12 +178,19 @@ void enqueue_round_end(bool batch, Integrator kind, const Launch& L, const PathB
 
 void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint32_t* bins, uint32_t bin_cap,
                 uint32_t* freel, int sort, int grid, hipStream_t st);
-void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st);
+// Counters of one slot-round bounce.  par < 0: every fetch counter and
+// sh_count; par 0/1: the closest-hit counters, the any-hit counters of that
+// parity and shc[par], leaving those of the other parity to a shadow pass
+// still running on a second stream.
+void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st, int par = -1);
 void launch_ctl_clear(Ctl* ctl, hipStream_t st);
 void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue,
                           const Buffers& B, uint32_t refill, int grid, hipStream_t st);
+// par < 0: the rays counted in sh_count; par 0/1: in shc[par], with that
+// parity's fetch counters and, when given, its own stack spill area.
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         const Buffers& B, uint32_t refill, int grid, hipStream_t st);
+                         const Buffers& B, uint32_t refill, int grid, hipStream_t st, int par = -1,
+                         uint32_t* spill = nullptr);
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st);
 

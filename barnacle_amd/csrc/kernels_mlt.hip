@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_mlt_start(DevScene S, PathBuf P, MltBuf
 template <bool FRESH>
 __global__ __launch_bounds__(256) void k_shade_mlt(DevScene S, PathBuf P, MltBuf M, Ctl* ctl, const uint32_t* bins,
                                                    uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                                                   int depth, int max_depth, int rr_depth) {
+                                                   uint32_t* sh_ctr, int depth, int max_depth, int rr_depth) {
     uint32_t start[kBins + 1];
     start[0] = 0;
     for (int b = 0; b < kBins; ++b) start[b + 1] = start[b] + ctl->bin_count[b];
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_shade_mlt(DevScene S, PathBuf P, MltBuf
             P.L[slot] = f4(L, 0.0f);
         }
         wave_push_m(cont, &ctl->q_count[nxt], queue_next, slot);
-        wave_push_m(shadow, &ctl->sh_count, shq, slot);
+        wave_push_m(shadow, sh_ctr, shq, slot);
     }
 }
 
@@ -390,13 +390,14 @@ void launch_mlt_start(bool fresh, const DevScene& S, const PathBuf& P, const Mlt
 }
 void launch_shade_mlt(bool fresh, const DevScene& S, const PathBuf& P, const MltBuf& M, Ctl* ctl,
                       const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                      int depth, int max_depth, int rr_depth, int grid, hipStream_t st) {
+                      int depth, int max_depth, int rr_depth, int grid, hipStream_t st, int par) {
+    uint32_t* sh_ctr = par < 0 ? &ctl->sh_count : &ctl->shc[par];
     if (fresh)
         hipLaunchKernelGGL(k_shade_mlt<true>, dim3(grid), dim3(256), 0, st, S, P, M, ctl, bins, bin_cap, queue_next,
-                           nxt, shq, depth, max_depth, rr_depth);
+                           nxt, shq, sh_ctr, depth, max_depth, rr_depth);
     else
         hipLaunchKernelGGL(k_shade_mlt<false>, dim3(grid), dim3(256), 0, st, S, P, M, ctl, bins, bin_cap, queue_next,
-                           nxt, shq, depth, max_depth, rr_depth);
+                           nxt, shq, sh_ctr, depth, max_depth, rr_depth);
 }
 void launch_mlt_boot_done(const PathBuf& P, float* weights, int id0, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_mlt_boot_done, dim3(blocks(n)), dim3(256), 0, st, P, weights, id0, n);

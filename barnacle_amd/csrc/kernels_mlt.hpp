@@ -34,7 +34,7 @@ void launch_mlt_start(bool fresh, const DevScene& S, const PathBuf& P, const Mlt
                       Ctl* ctl, uint32_t* queue, int id0, int n, int grid, hipStream_t st);
 void launch_shade_mlt(bool fresh, const DevScene& S, const PathBuf& P, const MltBuf& M, Ctl* ctl,
                       const uint32_t* bins, uint32_t bin_cap, uint32_t* queue_next, int nxt, uint32_t* shq,
-                      int depth, int max_depth, int rr_depth, int grid, hipStream_t st);
+                      int depth, int max_depth, int rr_depth, int grid, hipStream_t st, int par = -1);
 void launch_mlt_boot_done(const PathBuf& P, float* weights, int id0, int n, hipStream_t st);
 void launch_mlt_replay_done(const PathBuf& P, const MltBuf& M, const MltParamsDev& p, int c0, int n, hipStream_t st);
 void launch_mlt_accept(const PathBuf& P, const MltBuf& M, const MltParamsDev& p, float* film, int n, hipStream_t st);

@@ -267,11 +267,19 @@ __global__ void k_ctl_clear(Ctl* ctl) {
     uint32_t* w = reinterpret_cast<uint32_t*>(ctl);
     if (threadIdx.x < sizeof(Ctl) / 4) w[threadIdx.x] = 0;
 }
-__global__ void k_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg) {
+__global__ void k_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, int par) {
     const int i = threadIdx.x;
-    if (i < 2 * kFetchSegs) fetch_seg[(size_t)i * kSegStride] = 0;
+    const bool mine = par < 0 ? i < kFetchSegs + 2 * kShadowSegs
+                              : i < kFetchSegs || (i >= kFetchSegs + par * kShadowSegs &&
+                                                   i < kFetchSegs + (par + 1) * kShadowSegs);
+    if (mine) fetch_seg[(size_t)i * kSegStride] = 0;
     if (i < kBins) ctl->bin_count[i] = 0;
-    if (i == 0) { ctl->sh_count = 0; ctl->q_count[nxt] = 0; ctl->free_count[nxt] = 0; }
+    if (i == 0) {
+        if (par < 0) ctl->sh_count = 0;
+        else ctl->shc[par] = 0;
+        ctl->q_count[nxt] = 0;
+        ctl->free_count[nxt] = 0;
+    }
     if (i < 4) ctl->fetch[i] = 0;
 }
 
@@ -818,7 +826,7 @@ __global__ __launch_bounds__(256, K == 16 ? (BATCH ? kTraceWavesBatch : kTraceWa
                                                       uint32_t* spill, size_t spill_stride, uint32_t refill,
                                                       unsigned long long* stats, uint32_t* seg) {
     __shared__ uint32_t stk[K * 256];
-    const uint32_t count = shq ? ctl->sh_count : ctl->shc[par];
+    const uint32_t count = shq && par < 0 ? ctl->sh_count : ctl->shc[par];
     TraceCount tc{0, 0, 0};
     persistent_trace<true, COUNT, K>(
         SegFetch{seg, kShadowSegs}, count, stk, spill, spill_stride, refill, tc,
@@ -1346,9 +1354,9 @@ static void launch_closest_t(bool count, int k, const Work<BATCH>& W, const Path
 // counters); spill: the stack spill area to use.
 template <bool BATCH>
 static void launch_shadow_t(bool count, int k, const Work<BATCH>& W, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                            float4* L_out, const Buffers& B, uint32_t refill, int grid, hipStream_t st, int par = 0,
+                            float4* L_out, const Buffers& B, uint32_t refill, int grid, hipStream_t st, int par = -1,
                             uint32_t* spill = nullptr) {
-    uint32_t* seg = B.fetch_seg + (size_t)(kFetchSegs + par * kShadowSegs) * kSegStride;
+    uint32_t* seg = B.fetch_seg + (size_t)(kFetchSegs + (par < 0 ? 0 : par) * kShadowSegs) * kSegStride;
     if (!spill) spill = B.spill;
 #define BN_TS(C, KK)                                                                                            \
     hipLaunchKernelGGL((k_trace_shadow<C, BATCH, KK>), dim3(grid), dim3(256), 0, st, W, P, ctl, shq, L_out, par, \
@@ -1569,8 +1577,8 @@ void launch_bin(const PathBuf& P, Ctl* ctl, int cur, const uint32_t* queue, uint
                 uint32_t* freel, int sort, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_bin, dim3(grid), dim3(256), 0, st, P, ctl, cur, queue, bins, bin_cap, freel, sort);
 }
-void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(2 * kFetchSegs), 0, st, ctl, nxt, fetch_seg);
+void launch_reset(Ctl* ctl, int nxt, uint32_t* fetch_seg, hipStream_t st, int par) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(2 * kFetchSegs), 0, st, ctl, nxt, fetch_seg, par);
 }
 void launch_ctl_clear(Ctl* ctl, hipStream_t st) {
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(64), 0, st, ctl);
@@ -1583,11 +1591,11 @@ void launch_trace_closest(bool count, const DevScene& S, const PathBuf& P, Ctl* 
     launch_closest_t<false>(count, B.stack_k, W, P, ctl, cur, queue, B, refill, grid, st);
 }
 void launch_trace_shadow(bool count, const DevScene& S, const PathBuf& P, Ctl* ctl, const uint32_t* shq,
-                         const Buffers& B, uint32_t refill, int grid, hipStream_t st) {
+                         const Buffers& B, uint32_t refill, int grid, hipStream_t st, int par, uint32_t* spill) {
     Work<false> W;
     W.S = S;
     W.it = RoundItem{};
-    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, nullptr, B, refill, grid, st);
+    launch_shadow_t<false>(count, B.stack_k, W, P, ctl, shq, nullptr, B, refill, grid, st, par, spill);
 }
 void launch_first_hit(const DevScene& S, int w, int h, int sample_id, int spp, int frame_id, int* inst, int* prim,
                       float* t, uint32_t* spill, size_t spill_stride, uint32_t* fetch, int grid, hipStream_t st) {

@@ -115,3 +115,34 @@ def test_pssmlt_config4_geometry_matches_oracle(ctx, orc):
     ok = ~gnan.any(axis=1)
     g, o = gfilm[ok], ofilm[ok]
     assert np.sqrt(np.sum((g - o) ** 2) / np.sum(o ** 2)) <= 1e-5
+
+
+def test_pssmlt_shadow_stream_overlap_changes_nothing(ctx):
+    """The shadow rays of each bounce traced on a second stream beside the
+    next bounce's closest-hit traversal (the default) or after shading on
+    one stream: the same bootstrap luminances, B and per-chain accept counts
+    (each path adds its terms in the same order), over several chain chunks."""
+    info = configs.caustic_scene(width=48, height=48, spp=8, max_depth=16, freq=16,
+                                 n_bootstrap=1 << 13, n_chains=1 << 10)
+    sc = scene.load_scene(info)
+    p = sc.mlt_params()
+    ds = ctx.scene(sc)
+    out = {}
+    ctx.set_option("mlt_chunk", 300)
+    try:
+        for on in (0, 1):
+            ctx.set_option("overlap_shadow", on)
+            film, st = ds.render_pssmlt(p)
+            boot, acc = ds.pssmlt_last(p)
+            out[on] = (film, st, boot, acc)
+    finally:
+        ctx.set_option("overlap_shadow", 1)
+        ctx.set_option("mlt_chunk", 1 << 20)
+    (f0, s0, b0, a0), (f1, s1, b1, a1) = out[0], out[1]
+    assert np.array_equal(b0.view(np.int32), b1.view(np.int32))
+    assert s0 == s1 and s0["b"] > 0
+    assert np.array_equal(a0, a1)
+    n0, n1 = ~np.isfinite(f0), ~np.isfinite(f1)
+    assert np.array_equal(n0, n1)
+    ok = ~n0.any(axis=1)
+    assert np.sqrt(np.sum((f0[ok] - f1[ok]) ** 2) / np.sum(f0[ok] ** 2)) <= 1e-5
