@@ -10,21 +10,18 @@
 #include "kernels.hpp"
 #include "kernels_mlt.hpp"
 #include "shading.hpp"
+#include "wave.hpp"
 
 namespace bnd {
 
-__device__ __forceinline__ uint32_t lane_id_m() { return __lane_id(); }
-__device__ __forceinline__ uint32_t mbcnt_m(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 __device__ __forceinline__ void wave_push_m(bool pred, uint32_t* ctr, uint32_t* arr, uint32_t val) {
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     const int leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
-    if ((int)lane_id_m() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    if ((int)lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
     base = __shfl(base, leader);
-    if (pred) arr[base + mbcnt_m(m)] = val;
+    if (pred) arr[base + mbcnt(m)] = val;
 }
 
 // ErfInv of PSSMLT.fs:69-94 (Giles 2010 single-precision polynomial).
