@@ -577,7 +577,7 @@ def main():
             return json.load(fh)
     prof = load_profile("counters_config1.json").get("kernels", {})
     names = {"trace_closest": "k_trace_closest<false, false, 16>", "trace_shadow": "k_trace_shadow<false, false, 16>",
-             "shade_c": "k_shade_c<false>"}
+             "shade_c": "k_shade_c<false, true>"}
     traffic_of = {k: prof.get(v, {}).get("hbm_bytes_per_launch") for k, v in names.items()}
     l2 = {k: {"l2_hit_rate": prof.get(v, {}).get("l2_hit_rate")} for k, v in names.items()}
 
