@@ -510,6 +510,7 @@ void render_frames(Ctx* c, SceneRec& sc, const bn_pt_params& p, bnd::Integrator 
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.octant_order = c->octant_order;
+    L.skip_dark = c->skip_dark;
     L.stream = s;
     if (c->overlap_shadow) {
         L.stream2 = c->stream2;
@@ -623,6 +624,7 @@ void backward_pt(Ctx* c, SceneRec& sc, const bn_pt_params& p, const float* d_fil
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.octant_order = c->octant_order;
+    L.skip_dark = c->skip_dark;
     L.stream = s;
     L.n_pix = (uint32_t)(p.width * p.height);
     L.n_slots = (uint32_t)slots;
@@ -729,6 +731,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
     L.grid_shade = c->n_cu * 8;
     L.refill = (uint32_t)c->refill;
     L.octant_order = c->octant_order;
+    L.skip_dark = c->skip_dark;
     L.stream = s;
     L.n_slots = (uint32_t)slots;
     const bnd::Buffers B = buffers(c);
@@ -740,7 +743,7 @@ void render_batch(Ctx* c, const bn_batch_item* items, int n, float* const* films
         std::string key = "block:" + std::to_string(L.grid_trace) + ":" + std::to_string(L.refill) + ":" +
                           std::to_string(B.stack_k) + ":" + std::to_string(L.n_slots) + ":" +
                           std::to_string((uintptr_t)c->b_items.p) + ":" + std::to_string((uintptr_t)c->b_scenes.p) +
-                          ":" + std::to_string((uintptr_t)c->L_out.p) + ":" + std::to_string((int)L.octant_order);
+                          ":" + std::to_string((uintptr_t)c->L_out.p) + ":" + std::to_string((int)L.octant_order) + ":" + std::to_string(L.skip_dark);
         if (Lr.uniform) {
             const bnd::DevScene& d = Lr.scene;
             for (const void* q : {(const void*)d.tlas, (const void*)d.blas, (const void*)d.tris, (const void*)d.insts,
@@ -1502,6 +1505,9 @@ extern "C" int bn_set_option(bn_ctx* ctx, const char* key, int64_t value) {
         } else if (k == "bvh_gpu") {
             if (value < 0 || value > 1) throw Error(BN_E_INVALID, "bvh_gpu must be 0 or 1");
             c->bvh_gpu = value != 0;
+        } else if (k == "skip_dark") {
+            if (value < 0 || value > 2) throw Error(BN_E_INVALID, "skip_dark must be 0, 1 or 2");
+            c->skip_dark = (int)value;
         } else if (k == "octant_order") {
             if (value < 0 || value > 1) throw Error(BN_E_INVALID, "octant_order must be 0 or 1");
             c->octant_order = value != 0;

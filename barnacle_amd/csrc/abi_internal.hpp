@@ -83,6 +83,7 @@ struct Ctx {
     hipStream_t stream2 = nullptr;        // shadow rays of compact rounds, beside `stream`
     hipEvent_t ev_shade = nullptr, ev_shadow = nullptr;
     bool octant_order = true;             // compact shading groups each block's outputs by ray octant
+    int skip_dark = 1;                    // no shadow ray for an exactly-zero NEE term: 0, 1 plain frames, 2 all
     bool overlap_shadow = true;           // trace shadow rays on stream2 in plain path-tracing frames
     int trace_blocks_per_cu = 7;
     int refill = 40;                      // idle lanes that trigger a refill

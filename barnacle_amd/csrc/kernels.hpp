@@ -111,6 +111,10 @@ struct Launch {
     // Compact shading groups each block's continuing paths and shadow rays
     // by the octant of their ray direction.
     bool octant_order;
+    // Compact shading casts no shadow ray whose NEE term is exactly zero:
+    // 0 never, 1 in plain frames (counting frames cast the reference's
+    // rays), 2 in every frame (counting frames count the rays traced).
+    int skip_dark;
 };
 
 // Path-replay adjoint state (section "adjoint" of kernels_pt.hip).

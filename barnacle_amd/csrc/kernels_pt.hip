@@ -510,6 +510,12 @@ struct Vertex {
     float4 sh_o, sh_d, nee_a, nee_b;  // shadow ray (sh_o.w t max) and NEE factors, when shadow
 };
 struct ShadeStage;
+// One channel of an NEE term a*b that adds exactly nothing: an operand is
+// zero and the other finite, so fma(a, b, L) == L for every L the radiance
+// can hold (it starts at +0 and no fma of it gives -0).
+__device__ __forceinline__ bool dark_term(float a, float b) {
+    return (a == 0.0f && __builtin_isfinite(b)) || (b == 0.0f && __builtin_isfinite(a));
+}
 // Lacc: the path's radiance accumulator (compact rounds), read and written
 // only when the vertex emits; null: the radiance is P.L[at], returned in v.L.
 // stage: the shadow ray and NEE factors go to LDS slot `slot` as soon as they
@@ -519,7 +525,7 @@ __device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const 
                                              const float4& a, const float4& b);
 __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem& it, const PathBuf& P, uint32_t at,
                                              int4 h, Vertex& v, float4* Lacc = nullptr, ShadeStage* stage = nullptr,
-                                             int slot = 0) {
+                                             int slot = 0, bool skip_dark = false) {
     const float4 o4 = P.o[at], d4 = P.d[at];
     const Ray r = {xyz(o4), xyz(d4)};
     const float prev_pdf = d4.w;
@@ -558,7 +564,7 @@ __device__ __forceinline__ void shade_vertex(const DevScene& S, const RoundItem&
             } else {
                 v.sh_o = so; v.sh_d = sd; v.nee_a = na; v.nee_b = nb;
             }
-            v.shadow = true;
+            v.shadow = !(skip_dark && dark_term(na.x, nb.x) && dark_term(na.y, nb.y) && dark_term(na.z, nb.z));
         }
         const float ul = lcg(rng);
         V2 ub;
@@ -655,7 +661,7 @@ __device__ __forceinline__ void stage_shadow(ShadeStage* stage, int slot, const 
 template <bool BATCH, bool ORDER>
 __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& X, const PathState& Y, float4* L_out,
                                            ShadeStage& sg, uint32_t base, uint32_t total, uint32_t& shaded,
-                                           uint32_t* const (&ctr)[2]) {
+                                           uint32_t* const (&ctr)[2], bool skip_dark) {
     const int t = (int)threadIdx.x;
     const uint32_t i = base + threadIdx.x;
     bool cont = false, shadow = false;
@@ -670,7 +676,7 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
         v.shadow = false;
         if (h.x >= 0) {
             ++shaded;
-            shade_vertex(W.scene(k), W.item(k), X, i, h, v, &L_out[pid], &sg, t);
+            shade_vertex(W.scene(k), W.item(k), X, i, h, v, &L_out[pid], &sg, t, skip_dark);
         }
         cont = v.cont;
         shadow = v.shadow;
@@ -746,13 +752,13 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
 // counted in bin_count[0].
 template <bool BATCH, bool ORDER>
 __global__ __launch_bounds__(256, 3) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
-                                                    int nxt, float4* L_out) {
+                                                    int nxt, float4* L_out, int skip_dark) {
     __shared__ ShadeStage sg;
     const uint32_t total = ctl->q_count[cur];
     uint32_t shaded = 0;
     uint32_t* const ctr[2] = {&ctl->q_count[nxt], &ctl->shc[cur]};
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
-        shade_tile<BATCH, ORDER>(W, X, Y, L_out, sg, base, total, shaded, ctr);
+        shade_tile<BATCH, ORDER>(W, X, Y, L_out, sg, base, total, shaded, ctr, skip_dark != 0);
     for (int off = 32; off > 0; off >>= 1) shaded += __shfl_xor(shaded, off);
     if (lane_id() == 0 && shaded) atomicAdd(&ctl->bin_count[0], shaded);
 }
@@ -1376,12 +1382,13 @@ static void compact_iterations_t(const Work<BATCH>& W, const Launch& L, const Pa
               launch_closest_t<BATCH>(count, B.stack_k, W, X, ctl, cur, nullptr, B, L.refill, L.grid_trace, st); }
         if (overlap) (void)hipStreamWaitEvent(st, L.ev_shadow, 0);
         { Bracket b(hook, "shade", st);
+          const int skip = L.skip_dark == 2 || (L.skip_dark == 1 && !count);
           if (L.octant_order)
               hipLaunchKernelGGL((k_shade_c<BATCH, true>), dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl,
-                                 cur, nxt, B.L_out);
+                                 cur, nxt, B.L_out, skip);
           else
               hipLaunchKernelGGL((k_shade_c<BATCH, false>), dim3(grid_c), dim3(256), 0, st, W, X, state_of(Y), ctl,
-                                 cur, nxt, B.L_out); }
+                                 cur, nxt, B.L_out, skip); }
         if (overlap) {
             (void)hipEventRecord(L.ev_shade, st);
             (void)hipStreamWaitEvent(st2, L.ev_shade, 0);

@@ -334,7 +334,9 @@ def adjoint(ctx, steps=200):
     (tm, tl), _, _ = configs.config3_params(sc, info)
     ds.set_params(tm, tl)
     target = torch.from_numpy(ds.render_pt(p))
-    _, st_fwd = ds.render_pt(p, stats=True)                 # counts of the timed forward render
+    ctx.set_option("skip_dark", 2)
+    _, st_fwd = ds.render_pt(p, stats=True)                 # rays the timed forward render traces
+    ctx.set_option("skip_dark", 1)
     fwd = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -359,13 +361,17 @@ def adjoint(ctx, steps=200):
     iters = np.diff(np.asarray(stamps))[1:]                 # steady state: the first step allocates
     it_s = float(np.median(iters))
     f_s = float(np.median(fwd))
-    # An iteration renders forward and then replays the same paths for the
-    # gradients (the same extension and shadow rays), so its algorithmic bytes
-    # are twice those of a forward render, counted at the last step's parameters.
+    # An iteration renders forward (no shadow ray whose NEE term is exactly
+    # zero) and then replays the same paths for the gradients (every shadow
+    # ray the reference casts: a zero NEE term can still have a gradient), at
+    # the last step's parameters.
+    ctx.set_option("skip_dark", 2)
+    _, st_it_fwd = ds.render_pt(p, stats=True)
+    ctx.set_option("skip_dark", 1)
     _, st_it = ds.render_pt(p, stats=True)
     paths = p.width * p.height * p.spp
     peak = HBM_PEAK_GBS * 1e9
-    f_bytes, it_bytes = frame_bytes(st_fwd), 2 * frame_bytes(st_it)
+    f_bytes, it_bytes = frame_bytes(st_fwd), frame_bytes(st_it_fwd) + frame_bytes(st_it)
     return {"config": "config3: config1 geometry, 4096 GGX sphere patches (albedo rgb, roughness) + 4096 "
                       "light-only back-wall tiles (emission rgb), 512x512, 32 spp, depth 8, 200 Adam steps, lr 1e-2",
             "optimisation": {"steps": steps, "seconds": t_opt, "seen_patches": int(seen.sum()),
@@ -540,10 +546,19 @@ def main():
     print(f"# scene: {sc.n_triangles} triangles, {params.width}x{params.height}x{params.spp}spp depth "
           f"{params.max_depth}", file=sys.stderr, flush=True)
 
-    # Counting pass (outside the timed region): exact per-frame work counts.
+    # Counting passes (outside the timed region): exact per-frame work counts.
+    # st: the rays the reference casts for this frame.  st_tr: the rays the
+    # GPU traces (plain frames cast no shadow ray whose NEE term is exactly
+    # zero: it could add nothing to the film).  `value` counts the former per
+    # second of frame time; the rooflines use the work actually traced.
     _, st = ds.render_pt(params, stats=True)
+    ctx.set_option("skip_dark", 2)
+    _, st_tr = ds.render_pt(params, stats=True)
+    ctx.set_option("skip_dark", 1)
     rays = st["rays_closest"] + st["rays_shadow"]
+    rays_tr = st_tr["rays_closest"] + st_tr["rays_shadow"]
     print(f"# counts: {json.dumps(st)}", file=sys.stderr, flush=True)
+    print(f"# traced: {json.dumps(st_tr)}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         ds.render_pt(params)
@@ -596,7 +611,7 @@ def main():
             r["traffic_gbs"] = traffic / (per_launch_ms * 1e-3) / 1e9
             r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
         return r
-    rl_closest = roofline("trace_closest", "trace_closest", closest_bytes(st))
+    rl_closest = roofline("trace_closest", "trace_closest", closest_bytes(st_tr))
     # The closest-hit traversal keeps 87% of its VALU issue slots busy
     # (profiles/trace_config1_sq_counters.md): the byte roofline is the
     # contract's yardstick, instruction issue is what limits it.
@@ -618,13 +633,18 @@ def main():
         "config": {"workload": "config1: Cornell box + 69,620-tri displaced icosphere (GGX), 1024x1024, "
                                f"{params.spp} spp, max depth {params.max_depth}, rr depth {params.rr_depth}",
                    "parallelism": "single GPU"},
+        "value_note": "value: the rays the reference casts for this frame (rays_per_frame) per second of frame "
+                      "time; the GPU traces rays_traced_per_frame (no shadow ray whose NEE term is exactly zero; "
+                      "film bit-identical), mrays_traced_per_s per second",
+        "mrays_traced_per_s": rays_tr * args.steps / elapsed / 1e6,
         "mpaths_per_s": params.width * params.height * params.spp * args.steps / elapsed / 1e6,
         "rays_per_frame": {"extension": st["rays_closest"], "shadow": st["rays_shadow"]},
-        "frame_algorithmic_bytes": frame_bytes(st),
-        "frame_hbm_roofline_frac": frame_bytes(st) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "rays_traced_per_frame": {"extension": st_tr["rays_closest"], "shadow": st_tr["rays_shadow"]},
+        "frame_algorithmic_bytes": frame_bytes(st_tr),
+        "frame_hbm_roofline_frac": frame_bytes(st_tr) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "roofline": rl_closest,
-        "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st)),
-                      "shade_c": roofline("shade_c", "shade", shade_bytes(st))},
+        "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st_tr)),
+                      "shade_c": roofline("shade_c", "shade", shade_bytes(st_tr))},
         "kernel_ms_per_step": {k: v[1] for k, v in kt.items()},
         "kernel_ms_note": "one frame with the kernels one after another and HIP events around each launch",
         "overlap_shadow": not args.no_overlap,

@@ -381,7 +381,14 @@ int bn_render_batch(bn_ctx* ctx, const bn_batch_item* items, int32_t n, float* c
  *   "octant_order"         1 (default): shading writes each workgroup's
  *                          continuing paths and shadow rays grouped by the
  *                          octant of their ray direction; 0: in thread order.
- *                          Films and counters are the same either way. */
+ *                          Films and counters are the same either way.
+ *   "skip_dark"            path-tracing frames cast no shadow ray whose NEE
+ *                          term is exactly zero (an operand zero, the other
+ *                          finite: the ray could add nothing).  0: every ray;
+ *                          1 (default): in plain frames, while counting
+ *                          frames (bn_trace_stats) cast and count the
+ *                          reference's rays; 2: counting frames count the
+ *                          rays traced.  Films are the same in every mode. */
 int bn_set_option(bn_ctx* ctx, const char* key, int64_t value);
 
 typedef struct bn_kernel_time {

@@ -116,3 +116,33 @@ def test_octant_order_in_batches(ctx):
     for r, a, b in zip(ref, got0, got1):
         assert np.array_equal(r.view(np.int32), a.view(np.int32))
         assert np.array_equal(r.view(np.int32), b.view(np.int32))
+
+
+@pytest.mark.parametrize("name", ["config1", "spheres", "batch7"])
+def test_dark_shadow_rays_skipped_keep_films(ctx, name):
+    """A shadow ray whose NEE term is exactly zero (an operand zero, the other
+    finite) adds nothing, so not casting it keeps the film bit for bit.  Mode
+    2 counts the rays traced: the same extension rays, fewer or equal shadow
+    rays than the reference casts (mode 1's counting frame)."""
+    info = {"config1": lambda: configs.config1(width=160, height=120, spp=8, freq=24),
+            "spheres": lambda: configs.sphere_scene(96, 64, 8, 8),
+            "batch7": lambda: configs.batch_scene(7, max_res=128, max_spp=8)}[name]()
+    sc = scene.load_scene(info)
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    print(f"skip_dark modes: {name}", flush=True)
+    try:
+        ctx.set_option("skip_dark", 0)
+        a = ds.render_pt(p)
+        ctx.set_option("skip_dark", 1)
+        b, s_ref = ds.render_pt(p, stats=True)
+        c = ds.render_pt(p)
+        ctx.set_option("skip_dark", 2)
+        d, s_tr = ds.render_pt(p, stats=True)
+    finally:
+        ctx.set_option("skip_dark", 1)
+    for f in (b, c, d):
+        assert np.array_equal(a.view(np.int32), f.view(np.int32))
+    assert s_tr["rays_closest"] == s_ref["rays_closest"]
+    assert s_tr["nodes_closest"] == s_ref["nodes_closest"]
+    assert 0 < s_tr["rays_shadow"] <= s_ref["rays_shadow"]
