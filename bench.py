@@ -633,15 +633,19 @@ def main():
         "config": {"workload": "config1: Cornell box + 69,620-tri displaced icosphere (GGX), 1024x1024, "
                                f"{params.spp} spp, max depth {params.max_depth}, rr depth {params.rr_depth}",
                    "parallelism": "single GPU"},
-        "value_note": "value: the rays the reference casts for this frame (rays_per_frame) per second of frame "
-                      "time; the GPU traces rays_traced_per_frame (no shadow ray whose NEE term is exactly zero; "
-                      "film bit-identical), mrays_traced_per_s per second",
+        "value_note": "value and frame_hbm_roofline_frac count the frame's work as the reference does it: its "
+                      "rays (rays_per_frame) and their algorithmic bytes per second of frame time.  The GPU "
+                      "traces rays_traced_per_frame (no shadow ray whose NEE term is exactly zero; film "
+                      "bit-identical): mrays_traced_per_s and frame_hbm_roofline_frac_traced count those, and "
+                      "the per-kernel rooflines count what each kernel traces",
         "mrays_traced_per_s": rays_tr * args.steps / elapsed / 1e6,
         "mpaths_per_s": params.width * params.height * params.spp * args.steps / elapsed / 1e6,
         "rays_per_frame": {"extension": st["rays_closest"], "shadow": st["rays_shadow"]},
         "rays_traced_per_frame": {"extension": st_tr["rays_closest"], "shadow": st_tr["rays_shadow"]},
-        "frame_algorithmic_bytes": frame_bytes(st_tr),
-        "frame_hbm_roofline_frac": frame_bytes(st_tr) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "frame_algorithmic_bytes": frame_bytes(st),
+        "frame_hbm_roofline_frac": frame_bytes(st) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "frame_algorithmic_bytes_traced": frame_bytes(st_tr),
+        "frame_hbm_roofline_frac_traced": frame_bytes(st_tr) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "roofline": rl_closest,
         "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st_tr)),
                       "shade_c": roofline("shade_c", "shade", shade_bytes(st_tr))},
