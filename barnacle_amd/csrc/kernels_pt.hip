@@ -749,7 +749,8 @@ __device__ __forceinline__ void shade_tile(const Work<BATCH>& W, const PathBuf& 
 // nee_a.w.  Blocks stride over the tiles and append in completion order (one
 // atomic per block and output; an order-preserving scan was measured: the
 // traversal of the next bounce ran no faster).  The vertices shaded are
-// counted in bin_count[0].
+// counted in bin_count[0].  skip_dark: no shadow ray is cast for an NEE term
+// that is exactly zero (dark_term), which leaves the film unchanged.
 template <bool BATCH, bool ORDER>
 __global__ __launch_bounds__(256, 3) void k_shade_c(Work<BATCH> W, PathBuf X, PathState Y, Ctl* ctl, int cur,
                                                     int nxt, float4* L_out, int skip_dark) {
