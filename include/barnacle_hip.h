@@ -286,8 +286,10 @@ int bn_scene_export_instances(bn_ctx* ctx, int32_t scene, bn_instance_upload* ou
 /* PathTracingIntegrator.Render (Extensions/Integrator/PathTracing.fs:14-81
  * through ProgressiveIntegrator.RenderTile/Render, Base/Integrator.fs:22-55):
  * writes film_rgb[W*H*3] in Film.Pixels order, row (H-y-1) first
- * (Base/Film.fs:41-46).  stats may be NULL; when given, a counting pass of
- * the traversal fills it (that pass is slower and not what is timed). */
+ * (Base/Film.fs:41-46).  film_rgb may be NULL: the frame is rendered into
+ * the context's device film and not copied to the host.  stats may be NULL;
+ * when given, a counting pass of the traversal fills it (that pass is slower
+ * and not what is timed). */
 int bn_render_pt(bn_ctx* ctx, int32_t scene, const bn_pt_params* params,
                  float* film_rgb, bn_trace_stats* stats);
 

@@ -146,3 +146,16 @@ def test_dark_shadow_rays_skipped_keep_films(ctx, name):
     assert s_tr["rays_closest"] == s_ref["rays_closest"]
     assert s_tr["nodes_closest"] == s_ref["nodes_closest"]
     assert 0 < s_tr["rays_shadow"] <= s_ref["rays_shadow"]
+
+
+def test_frame_left_on_device(ctx):
+    """bn_render_pt with a NULL film renders into the context's device film
+    without a host copy; the frames before and after are unaffected."""
+    sc = scene.load_scene(configs.config1(width=96, height=64, spp=4, freq=16))
+    ds = ctx.scene(sc)
+    p = sc.pt_params()
+    ref = ds.render_pt(p)
+    assert ds.render_pt(p, host=False) is None
+    _, st = ds.render_pt(p, stats=True, host=False)
+    assert st["rays_closest"] > 0
+    assert np.array_equal(ref.view(np.int32), ds.render_pt(p).view(np.int32))
