@@ -146,3 +146,28 @@ def test_pssmlt_shadow_stream_overlap_changes_nothing(ctx):
     assert np.array_equal(n0, n1)
     ok = ~n0.any(axis=1)
     assert np.sqrt(np.sum((f0[ok] - f1[ok]) ** 2) / np.sum(f0[ok] ** 2)) <= 1e-5
+
+
+@pytest.mark.parametrize("max_depth", [0, 1, 2])
+def test_pssmlt_shallow_depths_match_oracle(ctx, orc, max_depth):
+    """Depth 0 (no bounce: black film, B = 0, no chains run), 1 and 2 (one
+    and two bounces: the two-stream shadow schedule with a single parity and
+    with both): bootstrap, B and per-chain accept counts as the oracle's."""
+    info = configs.caustic_scene(width=32, height=32, spp=4, max_depth=max_depth, freq=8,
+                                 n_bootstrap=2048, n_chains=256, sigma=1.0 / 64.0, large_step=0.3)
+    sc = scene.load_scene(info)
+    p = sc.mlt_params()
+    ds = ctx.scene(sc)
+    gfilm, gst = ds.render_pssmlt(p)
+    gboot, gacc = ds.pssmlt_last(p)
+    ofilm, ost, oboot, oacc = orc.scene(sc).render_pssmlt(p)
+    assert np.array_equal(gboot.view(np.int32), oboot.view(np.int32))
+    assert gst["b"] == ost["b"]
+    assert np.array_equal(gacc, oacc)
+    assert gst["accepted"] == ost["accepted"] and gst["proposed"] == ost["proposed"]
+    gnan, onan = ~np.isfinite(gfilm), ~np.isfinite(ofilm)
+    assert np.array_equal(gnan, onan)
+    ok = ~gnan.any(axis=1)
+    g, o = gfilm[ok], ofilm[ok]
+    den = np.sum(o ** 2)
+    assert (np.sum(g ** 2) == 0.0) if den == 0 else np.sqrt(np.sum((g - o) ** 2) / den) <= 1e-5
